@@ -1,0 +1,214 @@
+/*
+ * gpufoam_amd — C ABI of the MI355X (gfx950) batched per-cell chemistry
+ * integrator.
+ *
+ * The library replaces the C++ seam FoamGpu::GpuKernelEvaluator and the
+ * host-side mechanism converters of gpuFoam with plain C entry points:
+ * opaque handles, caller-owned buffers, plain pointers and sizes, integer
+ * status codes instead of exit()/assert().  No torch or HIP types appear in
+ * the signatures; a stream is passed as `void*` (a hipStream_t, NULL for the
+ * default stream).
+ *
+ * Cell state layout (structure of arrays, one column per cell):
+ *   Y[k*ldY + cell]   k = 0..nSpecie-1  mass fractions
+ *   Y[nSpecie*ldY + cell]               temperature [K]
+ *   Y[(nSpecie+1)*ldY + cell]           pressure [Pa]
+ * so OpenFOAM's per-specie volScalarFields map onto rows without a repack.
+ * The reference packs the same numbers cell-major (AoS):
+ *   gpu_chemistry/src/gpuChemistryModels/gpuChemistryModel/gpuChemistryModel.C:34-53.
+ *
+ * All entry points return GFC_OK (0) on success; on failure they return a
+ * non-zero gfc_status and gfc_last_error() returns a message (thread local).
+ */
+#ifndef GPUFOAM_AMD_H
+#define GPUFOAM_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GFC_MAX_REACTION_WIDTH 10 /* gpuReaction.H:12 MAX_REACTION_WIDTH */
+#define GFC_MAX_SPECIES 300       /* gpuThirdBodyEfficiencies.H:8 effArray */
+#define GFC_NO_INT_EXPONENT 214748364 /* gpuLabelMax = INT_MAX/10 */
+
+typedef enum gfc_status {
+    GFC_OK = 0,
+    GFC_ERR_INVALID_ARGUMENT = 1,
+    GFC_ERR_PARSE = 2,           /* malformed dictionary text */
+    GFC_ERR_UNSUPPORTED = 3,     /* reaction type gpuFoam rejects */
+    GFC_ERR_MASS_IMBALANCE = 4,  /* makeGpuReactions.H:48-54 */
+    GFC_ERR_TOO_LONG = 5,        /* reaction side > 10 species */
+    GFC_ERR_TOO_MANY_SPECIES = 6,/* > 300 species */
+    GFC_ERR_EFFICIENCIES = 7,    /* efficiency count != nSpecie */
+    GFC_ERR_SOLVER_NAME = 8,     /* makeGpuOdeSolver.H:28 */
+    GFC_ERR_CAPACITY = 9,        /* nCells > maxCells of the evaluator */
+    GFC_ERR_HIP = 10,            /* HIP runtime error */
+    GFC_ERR_NOT_BUILT = 11,      /* no kernel for this mechanism size */
+    GFC_ERR_LTS = 12             /* local time stepping is not supported */
+} gfc_status;
+
+typedef enum gfc_rate_type {
+    GFC_RATE_ARRHENIUS = 0,          /* gpuArrheniusReactionRate */
+    GFC_RATE_THIRD_BODY = 1,         /* gpuThirdBodyArrheniusReactionRate */
+    GFC_RATE_LINDEMANN = 2,          /* FallOff<Arrhenius, Lindemann> */
+    GFC_RATE_TROE = 3                /* FallOff<Arrhenius, Troe> */
+} gfc_rate_type;
+
+typedef enum gfc_solver {
+    GFC_ROSENBROCK12 = 12,
+    GFC_ROSENBROCK23 = 23,
+    GFC_ROSENBROCK34 = 34
+} gfc_solver;
+
+/* Per-cell status bits written to the optional cellStatus array. */
+#define GFC_CELL_MAX_STEPS 0x1     /* odeSolve ran out of maxSteps (silent in the reference) */
+#define GFC_CELL_STEP_UNDERFLOW 0x2/* dx < vSmall (reference: printf + assert) */
+#define GFC_CELL_SKIPPED 0x4       /* T <= Treact, chemistry not solved */
+
+/* Flags of gfc_solve. */
+#define GFC_HOST_POINTERS 0x1 /* buffers are host memory; staged through
+                                 device buffers owned by the evaluator */
+#define GFC_BATCHED 0x2       /* persistent work-stealing launch */
+#define GFC_GRAPH 0x4         /* replay the launch as a hipGraph (batched) */
+
+/* odeCoeffs of chemistryProperties (readGpuOdeInputs.H:8-26). */
+typedef struct gfc_ode_inputs {
+    int solver; /* gfc_solver */
+    double absTol;
+    double relTol;
+    int maxSteps;
+    double safeScale;
+    double alphaInc;
+    double alphaDec;
+    double minScale;
+    double maxScale;
+} gfc_ode_inputs;
+
+/* Read-only view of a compiled mechanism (structure of arrays).  The
+ * species thermo coefficients are already multiplied by R = RR/W
+ * (makeGpuThermo.H:41-44); the per-reaction composite thermo (used for Kc)
+ * is the one built by set_reaction_thermo (makeGpuReactions.H:18-59). */
+typedef struct gfc_mech_view {
+    int n_species;
+    int n_reactions;
+    double Tlow;  /* reaction temperature limits (reactions dict Tlow/Thigh) */
+    double Thigh;
+    /* species [n_species] */
+    const double* W;
+    const double* sTlow;
+    const double* sThigh;
+    const double* sTcommon;
+    const double* high; /* [n_species*7] */
+    const double* low;  /* [n_species*7] */
+    /* reactions [n_reactions] */
+    const int* type;         /* gfc_rate_type */
+    const int* irreversible; /* 0/1 */
+    const double* k;         /* [nr*3] A, beta, Ta: the rate (Arrhenius,
+                                third body) or k0 (fall-off) */
+    const double* kInf;      /* [nr*3] fall-off kInf */
+    const double* troe;      /* [nr*4] alpha, Tsss, Ts, Tss */
+    const double* eff;       /* [nr*n_species] third-body efficiencies */
+    const int* n_lhs;
+    const int* n_rhs;
+    const int* lhs_index;       /* [nr*GFC_MAX_REACTION_WIDTH] */
+    const double* lhs_stoich;   /* idem */
+    const double* lhs_exp;      /* exponent as a scalar */
+    const int* lhs_exp_int;     /* integer exponent or GFC_NO_INT_EXPONENT */
+    const int* rhs_index;
+    const double* rhs_stoich;
+    const double* rhs_exp;
+    const int* rhs_exp_int;
+    /* composite reaction thermo [n_reactions] */
+    const double* rY;
+    const double* rW;
+    const double* rTcommon;
+    const double* rhigh; /* [nr*7] */
+    const double* rlow;  /* [nr*7] */
+} gfc_mech_view;
+
+typedef struct gfc_mechanism gfc_mechanism;
+typedef struct gfc_evaluator gfc_evaluator;
+
+/* Last error message of this thread ("" if none). */
+const char* gfc_last_error(void);
+
+/* ---- mechanism compiler (host) ----------------------------------------
+ * Replaces makeGpuThermos (makeGpuThermo.H:16-66), makeGpuReactions and
+ * make_reaction_rate (makeGpuReactions.H:18-142, makeGpuReactionRate.H:9-135).
+ * thermo_text: an OpenFOAM physicalProperties/thermo dictionary holding
+ *   `species N ( ... );` and one sub-dictionary per specie.
+ * reactions_text: an OpenFOAM reactions dictionary (`reactions { ... }`,
+ *   optional Tlow/Thigh). */
+int gfc_mechanism_from_foam(const char* thermo_text, const char* reactions_text,
+                            gfc_mechanism** out);
+void gfc_mechanism_destroy(gfc_mechanism* mech);
+int gfc_mechanism_view(const gfc_mechanism* mech, gfc_mech_view* out);
+/* Name of specie i (pointer valid while the mechanism lives). */
+const char* gfc_mechanism_species_name(const gfc_mechanism* mech, int i);
+/* Reaction string of reaction r as written ("2H + O2 = ..."). */
+const char* gfc_mechanism_reaction_equation(const gfc_mechanism* mech, int r);
+/* Raw (unscaled, per-mole/R) NASA-7 coefficients as read from the
+ * dictionary: high[7], low[7]. */
+int gfc_mechanism_raw_thermo(const gfc_mechanism* mech, int i, double* high7,
+                             double* low7);
+/* Index of a specie by name, -1 if absent. */
+int gfc_mechanism_species_index(const gfc_mechanism* mech, const char* name);
+
+/* ---- chemistryProperties ---------------------------------------------
+ * Reads odeCoeffs { solver ...; absTol ...; ... } and Treact from a
+ * chemistryProperties dictionary text with the reference's defaults
+ * (readGpuOdeInputs.H:12-22; gpuChemistryModel.C:69 Treact default 0). */
+int gfc_ode_inputs_from_foam(const char* chemistry_properties_text,
+                             gfc_ode_inputs* out, double* Treact);
+/* "Rosenbrock12|23|34" -> gfc_solver (makeGpuOdeSolver.H:13-31). */
+int gfc_solver_from_name(const char* name, int* solver);
+/* Defaults of readGpuOdeInputs.H:12-22 for the given solver. */
+void gfc_ode_inputs_default(int solver, gfc_ode_inputs* out);
+
+/* ---- evaluator (device) -----------------------------------------------
+ * Replaces GpuKernelEvaluator's constructor (gpuKernelEvaluator.cu:14-33):
+ * uploads the mechanism and pre-allocates every device buffer for up to
+ * maxCells cells, so that gfc_solve allocates nothing.  The device is the
+ * current HIP device at creation.  One evaluator per host thread. */
+int gfc_evaluator_create(const gfc_mechanism* mech, const gfc_ode_inputs* ode,
+                         double Treact, int maxCells, gfc_evaluator** out);
+void gfc_evaluator_destroy(gfc_evaluator* ev);
+
+/* One CFD time step of chemistry over nCells cells.
+ * Replaces GpuKernelEvaluator::computeRR / computeYNew
+ * (gpuKernelEvaluator.cu:35-97) and singleCellSolver (singleCellSolver.H:27-51).
+ *   deltaT         CFD time step
+ *   deltaTChemMax  upper clamp of the returned chemical time step
+ *   ldY            leading dimension (>= nCells) of Y0, Ynew, RR
+ *   rho    [nCells]            density (only read when RR != NULL)
+ *   Y0     [(nS+2)*ldY]        initial state (SoA, see above)
+ *   Ynew   [(nS+2)*ldY]        final state; may alias Y0; may be NULL
+ *   RR     [nS*ldY]            rho*(Ynew-Y0)/deltaT; may be NULL
+ *   deltaTChem [nCells]        in: chemical time step guess; out: new step
+ *                              clamped to deltaTChemMax
+ *   deltaTMin  [1]             out: min over cells of the new step before
+ *                              clamping (computeRR: :91-94); may be NULL
+ *   cellStats  [3*ldY]         optional: status bits, step attempts,
+ *                              accepted steps per cell
+ * Pointers are device pointers unless GFC_HOST_POINTERS is set.  Without
+ * GFC_HOST_POINTERS the call is asynchronous on `stream` and allocates
+ * nothing. */
+int gfc_solve(gfc_evaluator* ev, double deltaT, double deltaTChemMax,
+              int nCells, int ldY, const double* rho, const double* Y0,
+              double* Ynew, double* RR, double* deltaTChem, double* deltaTMin,
+              int* cellStats, void* stream, unsigned flags);
+
+/* Launch-shape control for the persistent batched form (the autotuner's
+ * search space).  cellsPerWave in {1,4,16,64} where the mechanism size
+ * allows; wavesPerBlock in {1,2,4}; chunk = cells taken per queue grab. */
+int gfc_evaluator_set_batched_shape(gfc_evaluator* ev, int cellsPerWave,
+                                    int wavesPerBlock, int chunk);
+int gfc_evaluator_get_batched_shape(const gfc_evaluator* ev, int* cellsPerWave,
+                                    int* wavesPerBlock, int* chunk);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPUFOAM_AMD_H */

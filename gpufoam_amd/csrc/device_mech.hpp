@@ -1,0 +1,168 @@
+// Device-resident mechanism tables and the per-specie / per-reaction
+// physics shared by every kernel.  All arithmetic is FP64 and follows the
+// operation order of gpuFoam's device code (paths below are relative to the
+// gpuFoam repository root).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "constants.hpp"
+
+namespace gfc {
+
+constexpr int kWidth = 10; // gpuReaction.H:12 MAX_REACTION_WIDTH
+constexpr int kNoIntExp = 214748364;
+
+// Flat device copy of gfc_mech_view plus host-precomputed constants.  Every
+// array is in one hipMalloc'd block owned by the evaluator.
+struct DeviceMech {
+    int nS;
+    int nR;
+    double Tlow, Thigh; // reaction temperature clip (gpuReaction.H:472-488)
+    // species [nS]
+    const double* W;
+    const double* R;       // RR / W
+    const double* Tcommon;
+    const double* high;    // [nS*7] R-scaled NASA-7
+    const double* low;     // [nS*7]
+    // reactions [nR]
+    const int* type;
+    const int* irreversible;
+    const double* k;    // [nR*3]
+    const double* kInf; // [nR*3]
+    const double* troe; // [nR*4]
+    const double* eff;  // [nR*nS]
+    const int* nl;
+    const int* nr;
+    const int* li;  // [nR*10]
+    const double* ls;
+    const double* le;
+    const int* lei;
+    const int* ri;
+    const double* rs;
+    const double* re;
+    const int* rei;
+    const double* rY;
+    const double* rW;
+    const double* rTcommon;
+    const double* rhigh; // [nR*7]
+    const double* rlow;  // [nR*7]
+};
+
+// ---------------------------------------------------------------- thermo --
+// gpuThermo.H:29-32
+__device__ __forceinline__ const double* coeffs_of(const double* high,
+                                                   const double* low,
+                                                   double Tcommon, double T) {
+    return (T < Tcommon) ? low : high;
+}
+// gpuThermo.H:177-184
+__device__ __forceinline__ double poly_Cp(const double* a, double T) {
+    return ((((a[4] * T + a[3]) * T + a[2]) * T + a[1]) * T + a[0]);
+}
+// gpuThermo.H:210-221
+__device__ __forceinline__ double poly_Ha(const double* a, double T) {
+    return (((((a[4] / 5.0 * T + a[3] / 4.0) * T + a[2] / 3.0) * T +
+              a[1] / 2.0) * T + a[0]) * T + a[5]);
+}
+// gpuThermo.H:333-338
+__device__ __forceinline__ double poly_dCpdT(const double* a, double T) {
+    return (((4.0 * a[4] * T + 3.0 * a[3]) * T + 2.0 * a[2]) * T + a[1]);
+}
+// gpuThermo.H:261-271 (the perfect-gas entropy term at pStd is zero)
+__device__ __forceinline__ double poly_S(const double* a, double T,
+                                         double logT) {
+    return ((((a[4] / 4.0 * T + a[3] / 3.0) * T + a[2] / 2.0) * T + a[1]) *
+                T + a[0] * logT + a[6]);
+}
+// gpuThermo.H:287-299
+__device__ __forceinline__ double poly_Gstd(const double* a, double T,
+                                            double logT) {
+    return ((a[0] * (1.0 - logT) -
+             (((a[4] / 20.0 * T + a[3] / 12.0) * T + a[2] / 6.0) * T +
+              a[1] / 2.0) * T - a[6]) * T + a[5]);
+}
+
+// ----------------------------------------------------------------- rates --
+// gpuArrheniusReactionRate.H:16-26
+__device__ __forceinline__ double arrhenius(const double* k, double T) {
+    double ak = k[0];
+    if (fabs(k[1]) > kVSmall) ak *= pow(T, k[1]);
+    if (fabs(k[2]) > kVSmall) ak *= exp(-k[2] / T);
+    return ak;
+}
+
+// gpuTroeFallOffFunction.H:27-47
+__device__ __forceinline__ double troe_F(const double* tr, double T,
+                                         double Pr) {
+    const double alpha = tr[0], Tsss = tr[1], Ts = tr[2], Tss = tr[3];
+    const double logPr = log10(fmax(Pr, kSmall));
+    const double Fcent = (1.0 - alpha) * exp(-T / Tsss) +
+                         alpha * exp(-T / Ts) + exp(-Tss / T);
+    const double logFcent = log10(fmax(Fcent, kSmall));
+    const double c = -0.4 - 0.67 * logFcent;
+    const double d = 0.14;
+    const double n = 0.75 - 1.27 * logFcent;
+    const double x1 = n - d * (logPr + c);
+    const double x2 = (logPr + c) / x1;
+    const double x3 = 1.0 + x2 * x2;
+    const double x4 = logFcent / x3;
+    return pow(10.0, x4);
+}
+
+// gpuSpecieExponent.H:21-36
+__device__ __forceinline__ double integer_pow(double x, int e) {
+    if (e == 0) return 1;
+    double xx = e > 0 ? x : 1 / x;
+    double y = 1;
+    for (int i = e; i != 0; i /= 2) {
+        if (i % 2) y *= xx;
+        xx *= xx;
+    }
+    return y;
+}
+// gpuSpecieExponent.H:103-110
+__device__ __forceinline__ double specie_pow(double x, int ei, double er) {
+    if (ei != kNoIntExp) return integer_pow(x, ei);
+    return pow(x, er);
+}
+__device__ __forceinline__ double specie_pow_m1(double x, int ei, double er) {
+    if (ei != kNoIntExp) return integer_pow(x, ei - 1);
+    return pow(x, er + (-1.0));
+}
+
+// Kc of reaction r at T (gpuThermo.H:80-113), as omega() evaluates it.
+__device__ __forceinline__ double reaction_Kc(const DeviceMech& m, int r,
+                                              double T) {
+    const double Y = m.rY[r], W = m.rW[r];
+    const double* a = coeffs_of(m.rhigh + 7 * r, m.rlow + 7 * r,
+                                m.rTcommon[r], T);
+    const double nm = Y / W;
+    const double arg = -Y * poly_Gstd(a, T, log(T)) / (kRR * T);
+    const double K = (arg < 600) ? exp(arg) : sqrt(kVGreat);
+    if (fabs(nm - kSmall) < kVSmall) return K;
+    return K * pow(kPstd / (kRR * T), nm);
+}
+
+// KcAnddKcTbyKc (gpuThermo.H:139-169)
+__device__ __forceinline__ void reaction_Kc_dKc(const DeviceMech& m, int r,
+                                                double T, double& Kc,
+                                                double& dKcdTbyKc) {
+    const double Y = m.rY[r], W = m.rW[r];
+    const double* a = coeffs_of(m.rhigh + 7 * r, m.rlow + 7 * r,
+                                m.rTcommon[r], T);
+    const double nm = Y / W;
+    const double logT = log(T);
+    const double Gstd = poly_Gstd(a, T, logT);
+    const bool nm_small = fabs(nm - kSmall) < kVSmall;
+    const double RRT = kRR * T;
+    const double arg = -Y * Gstd / RRT;
+    const double K = (arg < 600) ? exp(arg) : sqrt(kVGreat);
+    Kc = nm_small ? K : K * pow(kPstd / RRT, nm);
+    const double S = poly_S(a, T, logT);
+    double d = (S + Gstd / T) * Y / RRT;
+    if (!nm_small) d = d - nm / T;
+    dKcdTbyKc = d;
+}
+
+} // namespace gfc

@@ -1,0 +1,297 @@
+// The evaluator: device copy of the mechanism, pre-allocated buffers and
+// the gfc_solve entry point.  Replaces FoamGpu::GpuKernelEvaluator
+// (gpu_chemistry/src/gpuKernelEvaluator/gpuKernelEvaluator/gpuKernelEvaluator.cu:14-97)
+// and its scratch resource (gpuMemoryResource.cu:7-51): everything is
+// allocated once in gfc_evaluator_create, so a device-pointer solve
+// allocates nothing and only enqueues kernels on the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "evaluator.hpp"
+#include "mechanism.hpp"
+
+namespace gfc {
+
+__global__ void lane_solve_kernel(SolveArgs A, double* scratch, long stride);
+long lane_scratch_per_cell(int nS);
+
+namespace {
+
+__global__ void init_min_kernel(unsigned long long* bits) {
+    *bits = static_cast<unsigned long long>(__double_as_longlong(HUGE_VAL));
+}
+
+__global__ void finish_min_kernel(const unsigned long long* bits, double* out) {
+    *out = __longlong_as_double(static_cast<long long>(*bits));
+}
+
+bool hipFail(hipError_t e, const char* what) {
+    if (e == hipSuccess) return false;
+    setLastError(std::string(what) + ": " + hipGetErrorString(e));
+    return true;
+}
+
+template <class T>
+void put(std::vector<char>& blob, size_t& off, const T* src, size_t n) {
+    off = (off + 255) & ~size_t(255);
+    const size_t bytes = n * sizeof(T);
+    if (blob.size() < off + bytes) blob.resize(off + bytes);
+    if (n) std::memcpy(blob.data() + off, src, bytes);
+    off += bytes;
+}
+
+} // namespace
+
+int evaluator_upload(gfc_evaluator* ev, const gfc_mechanism* mech) {
+    const gfc_mech_view v = mech->view();
+    const int nS = v.n_species, nR = v.n_reactions;
+    std::vector<double> R(nS);
+    for (int i = 0; i < nS; ++i) R[i] = kRR / v.W[i];
+    // host blob with 256-byte aligned sections, then one upload
+    std::vector<char> blob;
+    size_t off = 0;
+    std::vector<size_t> offs;
+    auto add = [&](auto* p, size_t n) {
+        put(blob, off, p, n);
+        offs.push_back(off - n * sizeof(*p));
+    };
+    add(v.W, nS);
+    add(R.data(), nS);
+    add(v.sTcommon, nS);
+    add(v.high, 7 * nS);
+    add(v.low, 7 * nS);
+    add(v.type, nR);
+    add(v.irreversible, nR);
+    add(v.k, 3 * nR);
+    add(v.kInf, 3 * nR);
+    add(v.troe, 4 * nR);
+    add(v.eff, (size_t)nS * nR);
+    add(v.n_lhs, nR);
+    add(v.n_rhs, nR);
+    add(v.lhs_index, kWidth * nR);
+    add(v.lhs_stoich, kWidth * nR);
+    add(v.lhs_exp, kWidth * nR);
+    add(v.lhs_exp_int, kWidth * nR);
+    add(v.rhs_index, kWidth * nR);
+    add(v.rhs_stoich, kWidth * nR);
+    add(v.rhs_exp, kWidth * nR);
+    add(v.rhs_exp_int, kWidth * nR);
+    add(v.rY, nR);
+    add(v.rW, nR);
+    add(v.rTcommon, nR);
+    add(v.rhigh, 7 * nR);
+    add(v.rlow, 7 * nR);
+    if (hipFail(hipMalloc(&ev->mechBlock, blob.size()), "hipMalloc(mechanism)"))
+        return GFC_ERR_HIP;
+    if (hipFail(hipMemcpy(ev->mechBlock, blob.data(), blob.size(),
+                          hipMemcpyHostToDevice),
+                "hipMemcpy(mechanism)"))
+        return GFC_ERR_HIP;
+    char* base = static_cast<char*>(ev->mechBlock);
+    int q = 0;
+    auto at = [&](auto*& dst) {
+        dst = reinterpret_cast<std::remove_reference_t<decltype(dst)>>(base + offs[q++]);
+    };
+    DeviceMech& d = ev->dm;
+    d.nS = nS;
+    d.nR = nR;
+    d.Tlow = v.Tlow;
+    d.Thigh = v.Thigh;
+    at(d.W); at(d.R); at(d.Tcommon); at(d.high); at(d.low);
+    at(d.type); at(d.irreversible); at(d.k); at(d.kInf); at(d.troe); at(d.eff);
+    at(d.nl); at(d.nr);
+    at(d.li); at(d.ls); at(d.le); at(d.lei);
+    at(d.ri); at(d.rs); at(d.re); at(d.rei);
+    at(d.rY); at(d.rW); at(d.rTcommon); at(d.rhigh); at(d.rlow);
+    ev->hostMech = v; // pointers into the mechanism, used by host code
+    return GFC_OK;
+}
+
+} // namespace gfc
+
+using namespace gfc;
+
+extern "C" {
+
+int gfc_evaluator_create(const gfc_mechanism* mech, const gfc_ode_inputs* ode,
+                         double Treact, int maxCells, gfc_evaluator** out) {
+    if (!mech || !ode || !out || maxCells <= 0) {
+        setLastError("gfc_evaluator_create: bad argument");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    if (ode->solver != 12 && ode->solver != 23 && ode->solver != 34) {
+        setLastError("Invalid ODE Solver type");
+        return GFC_ERR_SOLVER_NAME;
+    }
+    *out = nullptr;
+    auto* ev = new gfc_evaluator();
+    ev->nS = mech->nS;
+    ev->nR = mech->nR;
+    ev->maxCells = maxCells;
+    ev->ode = *ode;
+    ev->Treact = Treact;
+    hipGetDevice(&ev->device);
+    int st = evaluator_upload(ev, mech);
+    if (st != GFC_OK) {
+        gfc_evaluator_destroy(ev);
+        return st;
+    }
+    // plain kernel scratch: one slot per cell, rounded to whole wavefronts
+    ev->laneStride = ((long)maxCells + 63) / 64 * 64;
+    ev->laneScratchBytes = sizeof(double) * lane_scratch_per_cell(ev->nS) * ev->laneStride;
+    const int nE = ev->nS + 2;
+    const size_t ld = ev->laneStride;
+    if (hipFail(hipMalloc(&ev->laneScratch, ev->laneScratchBytes), "hipMalloc(scratch)") ||
+        hipFail(hipMalloc(&ev->minBits, 256), "hipMalloc(min)") ||
+        hipFail(hipMalloc(&ev->sY0, sizeof(double) * nE * ld), "hipMalloc(staging)") ||
+        hipFail(hipMalloc(&ev->sYnew, sizeof(double) * nE * ld), "hipMalloc(staging)") ||
+        hipFail(hipMalloc(&ev->sRR, sizeof(double) * ev->nS * ld), "hipMalloc(staging)") ||
+        hipFail(hipMalloc(&ev->sRho, sizeof(double) * ld), "hipMalloc(staging)") ||
+        hipFail(hipMalloc(&ev->sDt, sizeof(double) * ld), "hipMalloc(staging)") ||
+        hipFail(hipMalloc(&ev->sDtMin, 256), "hipMalloc(staging)") ||
+        hipFail(hipMalloc(&ev->sStats, sizeof(int) * 3 * ld), "hipMalloc(staging)")) {
+        gfc_evaluator_destroy(ev);
+        return GFC_ERR_HIP;
+    }
+    st = batched_init(ev);
+    if (st != GFC_OK) {
+        gfc_evaluator_destroy(ev);
+        return st;
+    }
+    *out = ev;
+    return GFC_OK;
+}
+
+void gfc_evaluator_destroy(gfc_evaluator* ev) {
+    if (!ev) return;
+    batched_free(ev);
+    void* ptrs[] = {ev->mechBlock, ev->laneScratch, ev->minBits, ev->sY0,
+                    ev->sYnew, ev->sRR, ev->sRho, ev->sDt, ev->sDtMin,
+                    ev->sStats};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete ev;
+}
+
+int gfc_solve(gfc_evaluator* ev, double deltaT, double deltaTChemMax,
+              int nCells, int ldY, const double* rho, const double* Y0,
+              double* Ynew, double* RR, double* deltaTChem, double* deltaTMin,
+              int* cellStats, void* stream, unsigned flags) {
+    if (!ev || !Y0 || !deltaTChem || nCells < 0 || ldY < nCells ||
+        !(deltaT > 0) || (RR && !rho)) {
+        setLastError("gfc_solve: bad argument");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    if (nCells > ev->maxCells) {
+        setLastError("gfc_solve: nCells exceeds the evaluator's maxCells");
+        return GFC_ERR_CAPACITY;
+    }
+    if (nCells == 0) {
+        if (deltaTMin) {
+            if (flags & GFC_HOST_POINTERS) *deltaTMin = HUGE_VAL;
+            else {
+                hipStream_t s = static_cast<hipStream_t>(stream);
+                hipLaunchKernelGGL(init_min_kernel, dim3(1), dim3(1), 0, s, ev->minBits);
+                hipLaunchKernelGGL(finish_min_kernel, dim3(1), dim3(1), 0, s,
+                                   ev->minBits, deltaTMin);
+            }
+        }
+        return GFC_OK;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int nS = ev->nS, nE = nS + 2;
+    const bool host = flags & GFC_HOST_POINTERS;
+
+    SolveArgs A{};
+    A.m = ev->dm;
+    A.tab = tableau(ev->ode.solver);
+    A.absTol = ev->ode.absTol;
+    A.relTol = ev->ode.relTol;
+    A.maxSteps = ev->ode.maxSteps;
+    A.safeScale = ev->ode.safeScale;
+    A.alphaInc = ev->ode.alphaInc;
+    A.alphaDec = ev->ode.alphaDec;
+    A.minScale = ev->ode.minScale;
+    A.maxScale = ev->ode.maxScale;
+    A.incThreshold = std::pow(ev->ode.maxScale / ev->ode.safeScale, -1.0 / ev->ode.alphaInc);
+    A.Treact = ev->Treact;
+    A.deltaT = deltaT;
+    A.deltaTChemMax = deltaTChemMax;
+    A.nCells = nCells;
+
+    if (host) {
+        // stage host buffers through the evaluator's device buffers
+        const long ld = nCells;
+        for (int k = 0; k < nE; ++k)
+            if (hipFail(hipMemcpyAsync(ev->sY0 + k * ld, Y0 + (long)k * ldY,
+                                       sizeof(double) * nCells,
+                                       hipMemcpyHostToDevice, s), "hipMemcpyAsync"))
+                return GFC_ERR_HIP;
+        if (hipFail(hipMemcpyAsync(ev->sDt, deltaTChem, sizeof(double) * nCells,
+                                   hipMemcpyHostToDevice, s), "hipMemcpyAsync"))
+            return GFC_ERR_HIP;
+        if (RR && hipFail(hipMemcpyAsync(ev->sRho, rho, sizeof(double) * nCells,
+                                         hipMemcpyHostToDevice, s), "hipMemcpyAsync"))
+            return GFC_ERR_HIP;
+        A.ldY = nCells;
+        A.rho = RR ? ev->sRho : nullptr;
+        A.Y0 = ev->sY0;
+        A.Ynew = Ynew ? ev->sYnew : nullptr;
+        A.RR = RR ? ev->sRR : nullptr;
+        A.dtChem = ev->sDt;
+        A.stats = cellStats ? ev->sStats : nullptr;
+    } else {
+        A.ldY = ldY;
+        A.rho = rho;
+        A.Y0 = Y0;
+        A.Ynew = Ynew;
+        A.RR = RR;
+        A.dtChem = deltaTChem;
+        A.stats = cellStats;
+    }
+    A.dtMinBits = deltaTMin ? ev->minBits : nullptr;
+    if (deltaTMin)
+        hipLaunchKernelGGL(init_min_kernel, dim3(1), dim3(1), 0, s, ev->minBits);
+
+    int st = GFC_OK;
+    if (flags & GFC_BATCHED) {
+        st = batched_launch(ev, A, s, (flags & GFC_GRAPH) != 0);
+        if (st != GFC_OK) return st;
+    } else {
+        const int blocks = (nCells + 63) / 64;
+        hipLaunchKernelGGL(lane_solve_kernel, dim3(blocks), dim3(64), 0, s, A,
+                           ev->laneScratch, ev->laneStride);
+    }
+    if (hipFail(hipGetLastError(), "kernel launch")) return GFC_ERR_HIP;
+    if (deltaTMin)
+        hipLaunchKernelGGL(finish_min_kernel, dim3(1), dim3(1), 0, s, ev->minBits,
+                           host ? ev->sDtMin : deltaTMin);
+
+    if (host) {
+        const long ld = nCells;
+        if (Ynew)
+            for (int k = 0; k < nE; ++k)
+                hipMemcpyAsync(Ynew + (long)k * ldY, ev->sYnew + k * ld,
+                               sizeof(double) * nCells, hipMemcpyDeviceToHost, s);
+        if (RR)
+            for (int k = 0; k < nS; ++k)
+                hipMemcpyAsync(RR + (long)k * ldY, ev->sRR + k * ld,
+                               sizeof(double) * nCells, hipMemcpyDeviceToHost, s);
+        hipMemcpyAsync(deltaTChem, ev->sDt, sizeof(double) * nCells,
+                       hipMemcpyDeviceToHost, s);
+        if (deltaTMin)
+            hipMemcpyAsync(deltaTMin, ev->sDtMin, sizeof(double), hipMemcpyDeviceToHost, s);
+        if (cellStats)
+            for (int k = 0; k < 3; ++k)
+                hipMemcpyAsync(cellStats + (long)k * ldY, ev->sStats + k * ld,
+                               sizeof(int) * nCells, hipMemcpyDeviceToHost, s);
+        if (hipFail(hipStreamSynchronize(s), "hipStreamSynchronize")) return GFC_ERR_HIP;
+    }
+    return st;
+}
+
+} // extern "C"

@@ -1,0 +1,44 @@
+// Internal definition of gfc_evaluator (the C ABI keeps it opaque).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/gpufoam_amd.h"
+#include "kernel_common.hpp"
+
+struct gfc_evaluator {
+    int device = 0;
+    int nS = 0;
+    int nR = 0;
+    int maxCells = 0;
+    gfc_ode_inputs ode{};
+    double Treact = 0;
+    gfc_mech_view hostMech{};
+    gfc::DeviceMech dm{};
+    void* mechBlock = nullptr;
+    // plain kernel: per-cell SoA scratch
+    double* laneScratch = nullptr;
+    long laneStride = 0;
+    size_t laneScratchBytes = 0;
+    unsigned long long* minBits = nullptr;
+    // staging for GFC_HOST_POINTERS
+    double* sY0 = nullptr;
+    double* sYnew = nullptr;
+    double* sRR = nullptr;
+    double* sRho = nullptr;
+    double* sDt = nullptr;
+    double* sDtMin = nullptr;
+    int* sStats = nullptr;
+    // persistent batched form
+    int cellsPerWave = 0;  // 0: chosen automatically
+    int wavesPerBlock = 0;
+    int chunk = 0;
+    void* batched = nullptr; // owned state of the batched launcher
+};
+
+namespace gfc {
+int batched_init(gfc_evaluator* ev);
+void batched_free(gfc_evaluator* ev);
+int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
+                   bool graph);
+} // namespace gfc

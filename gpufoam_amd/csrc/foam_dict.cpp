@@ -96,6 +96,14 @@ void parseInto(Dict& d, std::vector<Token>& toks, size_t& pos, bool nested) {
         Entry e;
         e.key = t.text;
         ++pos;
+        if (!e.key.empty() && e.key[0] == '#') {
+            // directive (#include "file", #includeEtc ...): one argument,
+            // no terminating ';'; kept as an entry, not expanded
+            if (pos < toks.size() && toks[pos].kind != Token::Punct)
+                e.tokens.push_back(toks[pos++]);
+            d.entries.push_back(std::move(e));
+            continue;
+        }
         if (pos < toks.size() && toks[pos].kind == Token::Punct &&
             toks[pos].text == "{") {
             ++pos;

@@ -1,0 +1,135 @@
+"""Host-side face of the chemistry evaluator.
+
+``Evaluator`` mirrors FoamGpu::GpuKernelEvaluator
+(gpu_chemistry/src/gpuKernelEvaluator/gpuKernelEvaluator/gpuKernelEvaluator.H:18-85):
+construct it once per mechanism with the odeCoeffs and Treact of
+chemistryProperties, then call ``computeRR`` (or ``computeYNew``) once per
+CFD time step.  The reference methods take and return host vectors with the
+state packed cell-major, [cell][Y..., T, p]; they are kept with the same
+meaning here.  ``solve`` is the direct, allocation-free entry for device
+tensors in the library's structure-of-arrays layout.
+
+torch is used only for device memory and streams; all chemistry runs in
+libgpufoam_amd's HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from .mechanism import Mechanism
+
+_vp = C.c_void_p
+
+
+def _ptr(a):
+    """Device (torch) or host (numpy) data pointer, None for None."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+class Evaluator:
+    def __init__(self, mech: Mechanism, ode: _capi.OdeInputs, Treact: float = 0.0,
+                 max_cells: int = 1 << 16):
+        self.mech = mech
+        self.ode = ode
+        self.Treact = Treact
+        self.max_cells = max_cells
+        self.nS = mech.n_species
+        self.nE = self.nS + 2
+        h = _vp()
+        _capi.check(_capi.lib().gfc_evaluator_create(mech.handle, C.byref(ode), Treact,
+                                                    max_cells, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _capi.lib().gfc_evaluator_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ----------------------------------------------------------- batched
+    def set_batched_shape(self, cells_per_wave: int, waves_per_block: int = 0,
+                          chunk: int = 0) -> None:
+        _capi.check(_capi.lib().gfc_evaluator_set_batched_shape(
+            self._h, cells_per_wave, waves_per_block, chunk))
+
+    def batched_shape(self) -> tuple[int, int, int]:
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _capi.check(_capi.lib().gfc_evaluator_get_batched_shape(
+            self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    # -------------------------------------------------------------- solve
+    def solve(self, deltaT: float, Y0, dtChem, Ynew=None, RR=None, rho=None,
+              deltaTMin=None, stats=None, deltaTChemMax: float = 1e300,
+              batched: bool = False, graph: bool = False, stream=None,
+              ldY: int | None = None) -> None:
+        """One CFD step on device tensors (SoA layout, see gpufoam_amd.h).
+
+        Y0 [nS+2, ld], Ynew like Y0 (may be Y0), RR [nS, ld], rho [n],
+        dtChem [n] (in/out), deltaTMin [1], stats int32 [3, ld].
+        Asynchronous on ``stream`` (default: torch's current stream)."""
+        n = dtChem.shape[0]
+        ld = ldY if ldY is not None else Y0.shape[1]
+        flags = 0
+        if batched:
+            flags |= _capi.BATCHED
+        if graph:
+            flags |= _capi.GRAPH
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        _capi.check(_capi.lib().gfc_solve(
+            self._h, deltaT, deltaTChemMax, n, ld, _ptr(rho), _ptr(Y0), _ptr(Ynew),
+            _ptr(RR), _ptr(dtChem), _ptr(deltaTMin), _ptr(stats), stream, flags))
+
+    def solve_host(self, deltaT: float, Y0: np.ndarray, dtChem: np.ndarray,
+                   rho: np.ndarray | None = None, deltaTChemMax: float = 1e300,
+                   want_rr: bool = False, batched: bool = False):
+        """Host-array solve (staged through the evaluator's device buffers).
+        Returns (Ynew, dtChem_new, deltaTMin, stats, RR)."""
+        Y0 = np.ascontiguousarray(Y0, dtype=np.float64)
+        n = Y0.shape[1]
+        Yn = np.zeros_like(Y0)
+        dt = np.array(dtChem, dtype=np.float64).copy()
+        dtmin = np.zeros(1)
+        stats = np.zeros((3, n), dtype=np.int32)
+        rr = np.zeros((self.nS, n)) if want_rr else None
+        rho_a = np.ascontiguousarray(rho if rho is not None else np.ones(n), dtype=np.float64)
+        flags = _capi.HOST_POINTERS | (_capi.BATCHED if batched else 0)
+        _capi.check(_capi.lib().gfc_solve(
+            self._h, deltaT, deltaTChemMax, n, n, _ptr(rho_a) if want_rr else None,
+            _ptr(Y0), _ptr(Yn), _ptr(rr), _ptr(dt), _ptr(dtmin), _ptr(stats), None, flags))
+        return Yn, dt, dtmin[0], stats, rr
+
+    # ------------------------------------------------ reference interface
+    def computeYNew(self, deltaT: float, deltaTChemMax: float, deltaTChem, Y):
+        """GpuKernelEvaluator::computeYNew (gpuKernelEvaluator.cu:35-63):
+        Y is the cell-major [nCells*(nS+2)] host vector; returns (YNew in the
+        same layout, deltaTChem new, unclamped as in the reference)."""
+        Yc = np.asarray(Y, dtype=np.float64).reshape(-1, self.nE)
+        Yn, dt, _, _, _ = self.solve_host(deltaT, Yc.T, deltaTChem)
+        return Yn.T.reshape(-1).copy(), dt
+
+    def computeRR(self, deltaT: float, deltaTChemMax: float, rho, deltaTChem, Y):
+        """GpuKernelEvaluator::computeRR (gpuKernelEvaluator.cu:65-97):
+        returns (RR cell-major [nCells*nS], deltaTChem new clamped to
+        deltaTChemMax, deltaTMin before clamping)."""
+        Yc = np.asarray(Y, dtype=np.float64).reshape(-1, self.nE)
+        _, dt, dtmin, _, rr = self.solve_host(deltaT, Yc.T, deltaTChem,
+                                              rho=np.asarray(rho, dtype=np.float64),
+                                              deltaTChemMax=deltaTChemMax, want_rr=True)
+        return rr.T.reshape(-1).copy(), dt, dtmin

@@ -47,6 +47,11 @@ struct DeviceMech {
     const double* rTcommon;
     const double* rhigh; // [nR*7]
     const double* rlow;  // [nR*7]
+    // wavefront kernel only: reactions are grouped in passes of 64;
+    // effP[(pass*nS + i)*64 + lane] is the efficiency of specie i for the
+    // reaction evaluated by `lane` in `pass`
+    const double* effP;
+    int nPass;
 };
 
 // ---------------------------------------------------------------- thermo --

@@ -6,7 +6,9 @@
 // allocates nothing and only enqueues kernels on the caller's stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -46,16 +48,43 @@ void put(std::vector<char>& blob, size_t& off, const T* src, size_t n) {
 
 } // namespace
 
-int evaluator_upload(gfc_evaluator* ev, const gfc_mechanism* mech) {
-    const gfc_mech_view v = mech->view();
+// Uploads the mechanism tables with the reactions in the order `perm`
+// (perm[r] = reference index of the reaction stored at r).
+static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
+                         DeviceMech& d, void*& block) {
     const int nS = v.n_species, nR = v.n_reactions;
     std::vector<double> R(nS);
     for (int i = 0; i < nS; ++i) R[i] = kRR / v.W[i];
-    // host blob with 256-byte aligned sections, then one upload
+    // reaction arrays gathered in `perm` order
+    auto gather = [&](auto* src, int width) {
+        using T = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
+        std::vector<T> out((size_t)nR * width);
+        for (int r = 0; r < nR; ++r)
+            for (int q = 0; q < width; ++q)
+                out[(size_t)r * width + q] = src[(size_t)perm[r] * width + q];
+        return out;
+    };
+    const auto type = gather(v.type, 1), irr = gather(v.irreversible, 1);
+    const auto k = gather(v.k, 3), kInf = gather(v.kInf, 3), troe = gather(v.troe, 4);
+    const auto eff = gather(v.eff, nS);
+    const auto nl = gather(v.n_lhs, 1), nr = gather(v.n_rhs, 1);
+    const auto li = gather(v.lhs_index, kWidth), lei = gather(v.lhs_exp_int, kWidth);
+    const auto ls = gather(v.lhs_stoich, kWidth), le = gather(v.lhs_exp, kWidth);
+    const auto ri = gather(v.rhs_index, kWidth), rei = gather(v.rhs_exp_int, kWidth);
+    const auto rs = gather(v.rhs_stoich, kWidth), re = gather(v.rhs_exp, kWidth);
+    const auto rY = gather(v.rY, 1), rW = gather(v.rW, 1), rTc = gather(v.rTcommon, 1);
+    const auto rhigh = gather(v.rhigh, 7), rlow = gather(v.rlow, 7);
+    const int nPass = (nR + 63) / 64;
+    std::vector<double> effP((size_t)nPass * nS * 64, 0.0);
+    for (int r = 0; r < nR; ++r) {
+        const int pass = r / 64, lane = r % 64;
+        for (int i = 0; i < nS; ++i)
+            effP[((size_t)pass * nS + i) * 64 + lane] = eff[(size_t)r * nS + i];
+    }
     std::vector<char> blob;
     size_t off = 0;
     std::vector<size_t> offs;
-    auto add = [&](auto* p, size_t n) {
+    auto add = [&](const auto* p, size_t n) {
         put(blob, off, p, n);
         offs.push_back(off - n * sizeof(*p));
     };
@@ -64,39 +93,38 @@ int evaluator_upload(gfc_evaluator* ev, const gfc_mechanism* mech) {
     add(v.sTcommon, nS);
     add(v.high, 7 * nS);
     add(v.low, 7 * nS);
-    add(v.type, nR);
-    add(v.irreversible, nR);
-    add(v.k, 3 * nR);
-    add(v.kInf, 3 * nR);
-    add(v.troe, 4 * nR);
-    add(v.eff, (size_t)nS * nR);
-    add(v.n_lhs, nR);
-    add(v.n_rhs, nR);
-    add(v.lhs_index, kWidth * nR);
-    add(v.lhs_stoich, kWidth * nR);
-    add(v.lhs_exp, kWidth * nR);
-    add(v.lhs_exp_int, kWidth * nR);
-    add(v.rhs_index, kWidth * nR);
-    add(v.rhs_stoich, kWidth * nR);
-    add(v.rhs_exp, kWidth * nR);
-    add(v.rhs_exp_int, kWidth * nR);
-    add(v.rY, nR);
-    add(v.rW, nR);
-    add(v.rTcommon, nR);
-    add(v.rhigh, 7 * nR);
-    add(v.rlow, 7 * nR);
-    if (hipFail(hipMalloc(&ev->mechBlock, blob.size()), "hipMalloc(mechanism)"))
+    add(type.data(), nR);
+    add(irr.data(), nR);
+    add(k.data(), 3 * nR);
+    add(kInf.data(), 3 * nR);
+    add(troe.data(), 4 * nR);
+    add(eff.data(), (size_t)nS * nR);
+    add(nl.data(), nR);
+    add(nr.data(), nR);
+    add(li.data(), kWidth * nR);
+    add(ls.data(), kWidth * nR);
+    add(le.data(), kWidth * nR);
+    add(lei.data(), kWidth * nR);
+    add(ri.data(), kWidth * nR);
+    add(rs.data(), kWidth * nR);
+    add(re.data(), kWidth * nR);
+    add(rei.data(), kWidth * nR);
+    add(rY.data(), nR);
+    add(rW.data(), nR);
+    add(rTc.data(), nR);
+    add(rhigh.data(), 7 * nR);
+    add(rlow.data(), 7 * nR);
+    add(effP.data(), effP.size());
+    if (hipFail(hipMalloc(&block, blob.size()), "hipMalloc(mechanism)"))
         return GFC_ERR_HIP;
-    if (hipFail(hipMemcpy(ev->mechBlock, blob.data(), blob.size(),
-                          hipMemcpyHostToDevice),
+    if (hipFail(hipMemcpy(block, blob.data(), blob.size(), hipMemcpyHostToDevice),
                 "hipMemcpy(mechanism)"))
         return GFC_ERR_HIP;
-    char* base = static_cast<char*>(ev->mechBlock);
+    char* base = static_cast<char*>(block);
     int q = 0;
     auto at = [&](auto*& dst) {
         dst = reinterpret_cast<std::remove_reference_t<decltype(dst)>>(base + offs[q++]);
     };
-    DeviceMech& d = ev->dm;
     d.nS = nS;
     d.nR = nR;
     d.Tlow = v.Tlow;
@@ -107,7 +135,29 @@ int evaluator_upload(gfc_evaluator* ev, const gfc_mechanism* mech) {
     at(d.li); at(d.ls); at(d.le); at(d.lei);
     at(d.ri); at(d.rs); at(d.re); at(d.rei);
     at(d.rY); at(d.rW); at(d.rTcommon); at(d.rhigh); at(d.rlow);
-    ev->hostMech = v; // pointers into the mechanism, used by host code
+    at(d.effP);
+    d.nPass = nPass;
+    return GFC_OK;
+}
+
+int evaluator_upload(gfc_evaluator* ev, const gfc_mechanism* mech) {
+    const gfc_mech_view v = mech->view();
+    const int nR = v.n_reactions;
+    // reference order for the per-lane kernel
+    std::vector<int> ident(nR);
+    for (int r = 0; r < nR; ++r) ident[r] = r;
+    int st = upload_tables(v, ident, ev->dm, ev->mechBlock);
+    if (st != GFC_OK) return st;
+    // wavefront kernel: reactions grouped by rate type so that a pass of 64
+    // reactions follows one code path
+    std::vector<int> perm = ident;
+    auto rank = [&](int r) {
+        return v.type[r] * 2 + (v.type[r] == GFC_RATE_ARRHENIUS ? v.irreversible[r] : 0);
+    };
+    std::stable_sort(perm.begin(), perm.end(),
+                     [&](int a, int b) { return rank(a) < rank(b); });
+    st = upload_tables(v, perm, ev->dmTeam, ev->mechBlockTeam);
+    if (st != GFC_OK) return st;
     return GFC_OK;
 }
 
@@ -169,7 +219,7 @@ int gfc_evaluator_create(const gfc_mechanism* mech, const gfc_ode_inputs* ode,
 void gfc_evaluator_destroy(gfc_evaluator* ev) {
     if (!ev) return;
     batched_free(ev);
-    void* ptrs[] = {ev->mechBlock, ev->laneScratch, ev->minBits, ev->sY0,
+    void* ptrs[] = {ev->mechBlock, ev->mechBlockTeam, ev->laneScratch, ev->minBits, ev->sY0,
                     ev->sYnew, ev->sRR, ev->sRho, ev->sDt, ev->sDtMin,
                     ev->sStats};
     for (void* p : ptrs)

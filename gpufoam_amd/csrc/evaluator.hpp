@@ -13,9 +13,10 @@ struct gfc_evaluator {
     int maxCells = 0;
     gfc_ode_inputs ode{};
     double Treact = 0;
-    gfc_mech_view hostMech{};
     gfc::DeviceMech dm{};
     void* mechBlock = nullptr;
+    gfc::DeviceMech dmTeam{}; // reactions grouped by rate type
+    void* mechBlockTeam = nullptr;
     // plain kernel: per-cell SoA scratch
     double* laneScratch = nullptr;
     long laneStride = 0;

@@ -41,9 +41,12 @@ struct BatchedState {
 
 using KernelFn = void (*)(SolveArgs, int*);
 
+// Row capacity of the wavefront kernel for n = nSpecie+1: exact sizes for
+// the bundled mechanisms, padded capacities otherwise.
 KernelFn team_kernel_for(int n, int& N) {
-    if (n <= 16) { N = 16; return team_solve_kernel<16>; }
-    if (n <= 56) { N = 56; return team_solve_kernel<56>; }
+    if (n == 11) { N = 11; return team_solve_kernel<11>; }
+    if (n == 54) { N = 54; return team_solve_kernel<54>; }
+    if (n <= 32) { N = 32; return team_solve_kernel<32>; }
     if (n <= 64) { N = 64; return team_solve_kernel<64>; }
     N = 0;
     return nullptr;
@@ -109,9 +112,12 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
     }
     int N = 0;
     KernelFn fn = team_kernel_for(n, N);
-    if (!fn) {
-        setLastError("no wavefront kernel for this mechanism size");
-        return GFC_ERR_NOT_BUILT;
+    if (!fn || ev->dmTeam.teamWidth > kTeamWidth) {
+        // mechanism outside the wavefront kernel's capacity: one lane per cell
+        const int blocks = (A.nCells + 63) / 64;
+        hipLaunchKernelGGL(lane_solve_kernel, dim3(blocks), dim3(64), 0, s, A,
+                           ev->laneScratch, ev->laneStride);
+        return hipFail(hipGetLastError(), "lane launch") ? GFC_ERR_HIP : GFC_OK;
     }
     SolveArgs At = A;
     At.m = ev->dmTeam;

@@ -52,6 +52,42 @@ struct DeviceMech {
     // reaction evaluated by `lane` in `pass`
     const double* effP;
     int nPass;
+    // natural-log form of the rate constants: lnA[2r] = ln|A| of k,
+    // lnA[2r+1] = ln|A| of kInf, sgnA likewise (+1/-1)
+    const double* lnA;
+    const double* sgnA;
+    // distinct third-body efficiency vectors: effSets[s*64 + i], and the
+    // set of each reaction (-1 for plain Arrhenius)
+    const double* effSets;
+    const int* effSet;
+    int nSets;
+    // wavefront kernel: one record per reaction, field-major within a pass
+    // so that the 64 lanes of a pass load each field in one coalesced access:
+    // recD[(pass*kRecD + f)*64 + lane], recI[(pass*kRecI + f)*64 + lane]
+    const double* recD;
+    const int* recI;
+    int teamWidth; // widest reaction side (the record holds kTeamWidth)
+};
+
+// Record layout of the wavefront kernel (at most kTeamWidth species per side).
+constexpr int kTeamWidth = 3;
+enum RecD {
+    rdLnA, rdBeta, rdTa, rdSgnA,       // k (Arrhenius / third body) or k0
+    rdLnAi, rdBetai, rdTai, rdSgnAi,   // kInf of fall-off rates
+    rdAlpha, rdTsss, rdTs, rdTss,      // Troe
+    rdDn,                              // moles of products - reactants
+    rdTcommon,                         // Tcommon of the composite thermo
+    rdLs, rdRs = rdLs + kTeamWidth,    // stoichiometric coefficients
+    rdLe = rdRs + kTeamWidth,          // real exponents
+    rdRe = rdLe + kTeamWidth,
+    kRecD = rdRe + kTeamWidth
+};
+enum RecI {
+    riType, riIrr, riSet, riNl, riNr,
+    riLi, riRi = riLi + kTeamWidth,    // specie indices
+    riLei = riRi + kTeamWidth,         // integer exponents
+    riRei = riLei + kTeamWidth,
+    kRecI = riRei + kTeamWidth
 };
 
 // ---------------------------------------------------------------- thermo --

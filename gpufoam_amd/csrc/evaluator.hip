@@ -81,6 +81,73 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
         for (int i = 0; i < nS; ++i)
             effP[((size_t)pass * nS + i) * 64 + lane] = eff[(size_t)r * nS + i];
     }
+    // rate constants in log form and de-duplicated efficiency vectors
+    std::vector<double> lnA(2 * (size_t)nR), sgnA(2 * (size_t)nR);
+    for (int r = 0; r < nR; ++r) {
+        for (int q = 0; q < 2; ++q) {
+            const double A = (q == 0 ? k : kInf)[3 * (size_t)r];
+            lnA[2 * r + q] = std::log(std::fabs(A));
+            sgnA[2 * r + q] = A < 0 ? -1.0 : 1.0;
+        }
+    }
+    std::vector<std::vector<double>> sets;
+    std::vector<int> effSet(nR, -1);
+    for (int r = 0; r < nR; ++r) {
+        if (type[r] == GFC_RATE_ARRHENIUS) continue;
+        std::vector<double> row(eff.begin() + (size_t)r * nS, eff.begin() + (size_t)(r + 1) * nS);
+        int s = -1;
+        for (size_t q = 0; q < sets.size(); ++q)
+            if (sets[q] == row) s = static_cast<int>(q);
+        if (s < 0) {
+            s = static_cast<int>(sets.size());
+            sets.push_back(row);
+        }
+        effSet[r] = s;
+    }
+    std::vector<double> effSets(std::max<size_t>(1, sets.size()) * 64, 0.0);
+    for (size_t q = 0; q < sets.size(); ++q)
+        for (int i = 0; i < nS && i < 64; ++i) effSets[q * 64 + i] = sets[q][i];
+    // wavefront-kernel records (only usable when no side is wider than
+    // kTeamWidth)
+    int teamWidth = 0;
+    for (int r = 0; r < nR; ++r) teamWidth = std::max({teamWidth, nl[r], nr[r]});
+    std::vector<double> recD((size_t)nPass * kRecD * 64, 0.0);
+    std::vector<int> recI((size_t)nPass * kRecI * 64, 0);
+    for (int r = 0; r < nR; ++r) {
+        const size_t pass = r / 64, lane = r % 64;
+        auto D = [&](int f) -> double& { return recD[(pass * kRecD + f) * 64 + lane]; };
+        auto I = [&](int f) -> int& { return recI[(pass * kRecI + f) * 64 + lane]; };
+        D(rdLnA) = lnA[2 * r];
+        D(rdBeta) = k[3 * r + 1];
+        D(rdTa) = k[3 * r + 2];
+        D(rdSgnA) = sgnA[2 * r];
+        D(rdLnAi) = lnA[2 * r + 1];
+        D(rdBetai) = kInf[3 * r + 1];
+        D(rdTai) = kInf[3 * r + 2];
+        D(rdSgnAi) = sgnA[2 * r + 1];
+        for (int q = 0; q < 4; ++q) D(rdAlpha + q) = troe[4 * r + q];
+        double dn = 0.0;
+        for (int q = 0; q < nl[r]; ++q) dn -= ls[kWidth * r + q];
+        for (int q = 0; q < nr[r]; ++q) dn += rs[kWidth * r + q];
+        D(rdDn) = dn;
+        D(rdTcommon) = rTc[r];
+        I(riType) = type[r];
+        I(riIrr) = irr[r];
+        I(riSet) = effSet[r] < 0 ? 0 : effSet[r];
+        I(riNl) = nl[r];
+        I(riNr) = nr[r];
+        for (int q = 0; q < kTeamWidth; ++q) {
+            const bool okl = q < nl[r], okr = q < nr[r];
+            D(rdLs + q) = okl ? ls[kWidth * r + q] : 0.0;
+            D(rdRs + q) = okr ? rs[kWidth * r + q] : 0.0;
+            D(rdLe + q) = okl ? le[kWidth * r + q] : 0.0;
+            D(rdRe + q) = okr ? re[kWidth * r + q] : 0.0;
+            I(riLi + q) = okl ? li[kWidth * r + q] : 0;
+            I(riRi + q) = okr ? ri[kWidth * r + q] : 0;
+            I(riLei + q) = okl ? lei[kWidth * r + q] : 0;
+            I(riRei + q) = okr ? rei[kWidth * r + q] : 0;
+        }
+    }
     std::vector<char> blob;
     size_t off = 0;
     std::vector<size_t> offs;
@@ -115,6 +182,12 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     add(rhigh.data(), 7 * nR);
     add(rlow.data(), 7 * nR);
     add(effP.data(), effP.size());
+    add(lnA.data(), lnA.size());
+    add(sgnA.data(), sgnA.size());
+    add(effSets.data(), effSets.size());
+    add(effSet.data(), effSet.size());
+    add(recD.data(), recD.size());
+    add(recI.data(), recI.size());
     if (hipFail(hipMalloc(&block, blob.size()), "hipMalloc(mechanism)"))
         return GFC_ERR_HIP;
     if (hipFail(hipMemcpy(block, blob.data(), blob.size(), hipMemcpyHostToDevice),
@@ -136,7 +209,15 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     at(d.ri); at(d.rs); at(d.re); at(d.rei);
     at(d.rY); at(d.rW); at(d.rTcommon); at(d.rhigh); at(d.rlow);
     at(d.effP);
+    at(d.lnA);
+    at(d.sgnA);
+    at(d.effSets);
+    at(d.effSet);
+    at(d.recD);
+    at(d.recI);
+    d.teamWidth = teamWidth;
     d.nPass = nPass;
+    d.nSets = static_cast<int>(sets.size());
     return GFC_OK;
 }
 

@@ -5,21 +5,25 @@
 // Rosenbrock step -> jacobian / LU / derivatives; see
 // gpu_chemistry/src/gpuOde/*.H) is spread over the lanes:
 //   * species-parallel: lane i owns component i of every state and stage
-//     vector (y, dydx0, dfdx, k1..k4), in registers;
-//   * reaction-parallel: lane l evaluates reactions l, l+64, ...; their
-//     contributions to dN/dt and to the Jacobian are accumulated in LDS
-//     with FP64 LDS atomics (the reactions are stored grouped by rate type
-//     so that a pass of 64 reactions runs one code path);
+//     vector (y, dydx0, dfdx, k1..k4), in registers; mixture sums are DPP
+//     reductions;
+//   * reaction-parallel: lane l evaluates reactions l, l+64, ... (stored
+//     grouped by rate type so a pass of 64 runs one code path).  Rate
+//     constants are evaluated as exp(ln A + beta ln T - Ta/T) and the
+//     reverse rate as exp(... - ln Kc) with ln T, 1/T and ln(pStd/RT)
+//     computed once per cell; third-body concentrations are DPP reductions
+//     over the mechanism's distinct efficiency vectors.  Contributions to
+//     dN/dt and to the Jacobian are accumulated in LDS with FP64 LDS
+//     atomics;
 //   * row-parallel LU: lane i holds row i of -J + I/(gamma*dx) in N
-//     registers; Crout with implicit scaling becomes right-looking
-//     elimination with the same per-entry operation sequence, the pivot
-//     row is broadcast with v_readlane, and the pivot choice (largest
-//     scaled value, ties to the last row position, ludecompose.H:9-92)
-//     uses a wavefront argmax over (value, position);
-//   * the Jacobian stays in LDS across rejected attempts (it only depends
-//     on y0), so a retry only re-factorises.
-// N is the compile-time row capacity (n = nSpecie+1 <= N <= 64); unused
-// rows are identity rows and are skipped with uniform branches.
+//     registers.  Crout with implicit scaling (ludecompose.H:9-92) becomes
+//     right-looking elimination with the same per-entry operation sequence;
+//     the pivot (largest scaled value, ties to the last row position) is a
+//     DPP argmax, the pivot row is broadcast with v_readlane;
+//   * the Jacobian stays in LDS across rejected attempts (it depends only on
+//     y0), so a retry only re-factorises.
+// N is the compile-time row capacity (n = nSpecie+1 <= N <= 64); rows
+// n..N-1 are identity rows.
 #include <hip/hip_runtime.h>
 
 #include "kernel_common.hpp"
@@ -29,6 +33,8 @@ namespace gfc {
 namespace {
 
 constexpr int kLanes = 64;
+constexpr int kMaxSets = 16;                     // efficiency vectors per LDS chunk
+constexpr double kLnRootVGreat = 354.25825000766; // ln(sqrt(DBL_MAX/10))
 
 __device__ __forceinline__ double readlane(double v, int l) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
@@ -36,20 +42,52 @@ __device__ __forceinline__ double readlane(double v, int l) {
     return __hiloint2double(hi, lo);
 }
 
+template <int CTRL, int ROW_MASK = 0xf, int BANK_MASK = 0xf>
+__device__ __forceinline__ double dpp(double old, double v) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(v),
+                                               CTRL, ROW_MASK, BANK_MASK, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(v),
+                                               CTRL, ROW_MASK, BANK_MASK, false);
+    return __hiloint2double(hi, lo);
+}
+
+template <int CTRL, int ROW_MASK = 0xf, int BANK_MASK = 0xf>
+__device__ __forceinline__ int dppi(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, BANK_MASK, false);
+}
+
+// Sum over the 64 lanes in a fixed tree (row_shr 1,2,4,8 inside rows of 16,
+// then row_bcast 15 and 31); the result is uniform.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) v += __shfl_xor(v, s);
-    return v;
+    v += dpp<0x111>(0.0, v);
+    v += dpp<0x112>(0.0, v);
+    v += dpp<0x114>(0.0, v);
+    v += dpp<0x118>(0.0, v);
+    v += dpp<0x142, 0xa>(0.0, v);
+    v += dpp<0x143, 0xc>(0.0, v);
+    return readlane(v, 63);
 }
 
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) v = fmax(v, __shfl_xor(v, s));
-    return v;
+    const double lo = -HUGE_VAL;
+    v = fmax(v, dpp<0x111>(lo, v));
+    v = fmax(v, dpp<0x112>(lo, v));
+    v = fmax(v, dpp<0x114>(lo, v));
+    v = fmax(v, dpp<0x118>(lo, v));
+    v = fmax(v, dpp<0x142, 0xa>(lo, v));
+    v = fmax(v, dpp<0x143, 0xc>(lo, v));
+    return readlane(v, 63);
 }
 
-__device__ __forceinline__ double uniform(double v) {
-    return readlane(v, 0);
+__device__ __forceinline__ int wave_max_i(int v) {
+    const int lo = -2147483647 - 1;
+    v = max(v, dppi<0x111>(lo, v));
+    v = max(v, dppi<0x112>(lo, v));
+    v = max(v, dppi<0x114>(lo, v));
+    v = max(v, dppi<0x118>(lo, v));
+    v = max(v, dppi<0x142, 0xa>(lo, v));
+    v = max(v, dppi<0x143, 0xc>(lo, v));
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
 template <int N>
@@ -59,52 +97,212 @@ struct TeamLDS {
     double c[kLanes];
     double dN[kLanes];
     double Ha[kLanes];
-    double colA[kLanes]; // rhoM / W_j
-    double colB[kLanes]; // rhoM * v_j
-    int pl[kLanes];      // pivot lane of LU step j
+    double colA[kLanes];            // rhoM / W_j
+    double colB[kLanes];            // rhoM * v_j
+    double M[kLanes];               // third-body concentration per set
+    double gL[kLanes], gH[kLanes];  // W_i Gstd_i with the low / high polynomial
+    double hL[kLanes], hH[kLanes];  // W_i Ha_i with the low / high polynomial
+    double Wd[kMaxSets * kLanes];   // dense Jacobian weights per set, specie
 };
 
-// Product of concentration powers of one reaction side, in written order
-// (lhsPowers + calcCf, gpuReaction.H:348-386).
-__device__ __forceinline__ double side_pow(const int* idx, const int* ei,
-                                           const double* er, int n,
-                                           const double* c) {
+// Per-cell temperature functions shared by every reaction.
+struct TFun {
+    double T, logT, invT, invRRT, L1; // L1 = ln(pStd/(RR T))
+};
+
+__device__ __forceinline__ TFun tfun(double T) {
+    TFun f;
+    f.T = T;
+    f.logT = log(T);
+    f.invT = 1.0 / T;
+    f.invRRT = 1.0 / (kRR * T);
+    f.L1 = log(kPstd * f.invRRT);
+    return f;
+}
+
+// One reaction of a pass, loaded from the field-major record.
+struct Rxn {
+    int type, irr, set, nl, nr;
+    int li[kTeamWidth], ri[kTeamWidth], lei[kTeamWidth], rei[kTeamWidth];
+    double lnA, beta, Ta, sgnA, lnAi, betai, Tai, sgnAi;
+    double troe[4];
+    double dn, Tcommon;
+    double ls[kTeamWidth], rs[kTeamWidth], le[kTeamWidth], re[kTeamWidth];
+};
+
+__device__ __forceinline__ Rxn load_rxn(const DeviceMech& m, int pass, int lane) {
+    const double* D = m.recD + (long)pass * kRecD * kLanes + lane;
+    const int* I = m.recI + (long)pass * kRecI * kLanes + lane;
+    Rxn x;
+    x.type = I[riType * kLanes];
+    x.irr = I[riIrr * kLanes];
+    x.set = I[riSet * kLanes];
+    x.nl = I[riNl * kLanes];
+    x.nr = I[riNr * kLanes];
+#pragma unroll
+    for (int q = 0; q < kTeamWidth; ++q) {
+        x.li[q] = I[(riLi + q) * kLanes];
+        x.ri[q] = I[(riRi + q) * kLanes];
+        x.lei[q] = I[(riLei + q) * kLanes];
+        x.rei[q] = I[(riRei + q) * kLanes];
+        x.ls[q] = D[(rdLs + q) * kLanes];
+        x.rs[q] = D[(rdRs + q) * kLanes];
+        x.le[q] = D[(rdLe + q) * kLanes];
+        x.re[q] = D[(rdRe + q) * kLanes];
+    }
+    x.lnA = D[rdLnA * kLanes];
+    x.beta = D[rdBeta * kLanes];
+    x.Ta = D[rdTa * kLanes];
+    x.sgnA = D[rdSgnA * kLanes];
+    x.lnAi = D[rdLnAi * kLanes];
+    x.betai = D[rdBetai * kLanes];
+    x.Tai = D[rdTai * kLanes];
+    x.sgnAi = D[rdSgnAi * kLanes];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) x.troe[q] = D[(rdAlpha + q) * kLanes];
+    x.dn = D[rdDn * kLanes];
+    x.Tcommon = D[rdTcommon * kLanes];
+    return x;
+}
+
+// x^e for the integer exponents of the mechanism, with the multiplication
+// order of integerPow (gpuSpecieExponent.H:21-36); real exponents use pow.
+__device__ __forceinline__ double ipow(double x, int e, double er) {
+    if (e == 1) return x;
+    if (e == 2) return x * x;
+    if (e == 3) return x * (x * x);
+    if (e == 0) return 1.0;
+    return specie_pow(x, e, er);
+}
+__device__ __forceinline__ double ipow_m1(double x, int e, double er) {
+    if (e == 1) return 1.0;
+    if (e == 2) return x;
+    if (e == 3) return x * x;
+    return specie_pow_m1(x, e, er);
+}
+
+// Concentration powers of one side (lhsPowers, gpuReaction.H:348-372) and
+// their product in written order (calcCf, :374-386).
+__device__ __forceinline__ double side_powers(const int* idx, const int* ei,
+                                              const double* er, int n,
+                                              const double* c, double* pw) {
     double C = 1.0;
-#pragma unroll 1
-    for (int q = 0; q < n; ++q) C *= specie_pow(c[idx[q]], ei[q], er[q]);
+#pragma unroll
+    for (int q = 0; q < kTeamWidth; ++q) {
+        pw[q] = (q < n) ? ipow(c[idx[q]], ei[q], er[q]) : 1.0;
+        if (q < n) C *= pw[q];
+    }
     return C;
 }
 
-// d(product)/dc_j, the product of the other factors times the derivative of
-// factor j (calcdCfdcj, gpuReaction.H:388-429)
+// d(product)/dc at position j (calcdCfdcj, :388-429)
 __device__ __forceinline__ double side_dpow(const int* idx, const int* ei,
                                             const double* er, int n, int j,
-                                            const double* c) {
+                                            const double* c, const double* pw) {
     double v = 1;
-#pragma unroll 1
-    for (int i = 0; i < n; ++i) {
-        const double ci = c[idx[i]];
-        if (i == j) v *= er[i] * specie_pow_m1(ci, ei[i], er[i]);
-        else v *= specie_pow(ci, ei[i], er[i]);
+#pragma unroll
+    for (int i = 0; i < kTeamWidth; ++i) {
+        if (i >= n) break;
+        if (i == j) v *= er[i] * ipow_m1(c[idx[i]], ei[i], er[i]);
+        else v *= pw[i];
     }
     return v;
 }
 
-// M = sum_i eff_i c_i for the reaction at (pass, lane)
-__device__ __forceinline__ double third_body_M(const DeviceMech& m, int pass,
-                                               int lane, const double* c) {
-    const double* e = m.effP + (long)pass * m.nS * kLanes + lane;
-    double M = 0;
-    for (int i = 0; i < m.nS; ++i) M += e[(long)i * kLanes] * c[i];
-    return M;
+// Third-body concentrations of every efficiency set: M[s] = sum_i eff_si c_i.
+template <int N>
+__device__ __forceinline__ void third_body(const DeviceMech& m, TeamLDS<N>& L,
+                                           double cl, int lane) {
+#pragma unroll 1
+    for (int s = 0; s < m.nSets; ++s) {
+        const double Ms = wave_sum(m.effSets[s * kLanes + lane] * cl);
+        if (lane == 0) L.M[s] = Ms;
+    }
+}
+
+// Specie Gibbs energies (and enthalpies) with both polynomial sets, times W:
+// a reaction's Y*Gstd of its composite thermo (gpuThermo.H:354-470,
+// makeGpuReactions.H:18-59) is the stoichiometric sum of these with the set
+// chosen by the composite's Tcommon.
+template <int N, bool ENTHALPY>
+__device__ __forceinline__ void specie_gibbs(const DeviceMech& m, TeamLDS<N>& L,
+                                             const TFun& f, int lane) {
+    if (lane < m.nS) {
+        const double W = m.W[lane];
+        const double* lo = m.low + 7 * lane;
+        const double* hi = m.high + 7 * lane;
+        L.gL[lane] = W * poly_Gstd(lo, f.T, f.logT);
+        L.gH[lane] = W * poly_Gstd(hi, f.T, f.logT);
+        if (ENTHALPY) {
+            L.hL[lane] = W * poly_Ha(lo, f.T);
+            L.hH[lane] = W * poly_Ha(hi, f.T);
+        }
+    }
+}
+
+// ln Kc (gpuThermo.H:80-113 in log form) and optionally dKcdTbyKc
+// (gpuThermo.H:139-169, with S + Gstd/T = Ha/T).
+template <int N>
+__device__ __forceinline__ double ln_kc(const Rxn& x, const TFun& f,
+                                        const TeamLDS<N>& L, double* dKc) {
+    const bool low = f.T < x.Tcommon;
+    const double* g = low ? L.gL : L.gH;
+    double sG = 0.0;
+#pragma unroll
+    for (int q = 0; q < kTeamWidth; ++q) {
+        if (q < x.nr) sG += x.rs[q] * g[x.ri[q]];
+        if (q < x.nl) sG -= x.ls[q] * g[x.li[q]];
+    }
+    const double arg = -sG * f.invRRT;
+    const double lnK = ((arg < 600) ? arg : kLnRootVGreat) + x.dn * f.L1;
+    if (dKc) {
+        const double* h = low ? L.hL : L.hH;
+        double sH = 0.0;
+#pragma unroll
+        for (int q = 0; q < kTeamWidth; ++q) {
+            if (q < x.nr) sH += x.rs[q] * h[x.ri[q]];
+            if (q < x.nl) sH -= x.ls[q] * h[x.li[q]];
+        }
+        *dKc = sH * f.invRRT * f.invT - x.dn * f.invT;
+    }
+    return lnK;
+}
+
+// Forward rate constant (gpuReactionRate.H:164-173; everything(),
+// :237-245) in log form.  kArr: the factor of the concentration derivative
+// (third body: the Arrhenius part; fall-off: none); ddT: dkf/dT.
+__device__ __forceinline__ double forward_rate(const Rxn& x, const TFun& f,
+                                               const double* Mset, double& kArr,
+                                               double& ddT) {
+    const double Ak = x.sgnA * exp(x.lnA + x.beta * f.logT - x.Ta * f.invT);
+    if (x.type == GFC_RATE_ARRHENIUS) {
+        kArr = Ak;
+        ddT = Ak * (x.beta + x.Ta * f.invT) * f.invT;
+        return Ak;
+    }
+    const double M = Mset[x.set];
+    if (x.type == GFC_RATE_THIRD_BODY) {
+        kArr = Ak;
+        ddT = M * (Ak * (x.beta + x.Ta * f.invT) * f.invT);
+        return M * Ak;
+    }
+    // fall-off (gpuFallOffReactionRate.H:39-158): the Jacobian keeps dkInf/dT only
+    const double kInf = x.sgnAi * exp(x.lnAi + x.betai * f.logT - x.Tai * f.invT);
+    const double Pr = Ak / kInf * M;
+    const double F = (x.type == GFC_RATE_TROE) ? troe_F(x.troe, f.T, Pr) : 1.0;
+    const double temp = (Pr / (1.0 + Pr)) * F;
+    kArr = 0.0;
+    ddT = (kInf * (x.betai + x.Tai * f.invT) * f.invT) * temp;
+    return kInf * temp;
 }
 
 // ---------------------------------------------------------- derivatives --
 // gpuODESystem::derivatives (gpuODESystem.H:164-196).  yl: component of
-// this lane; returns dydx component of this lane.
+// this lane; returns the dydx component of this lane.
 template <int N>
-__device__ __forceinline__ double team_derivatives(const DeviceMech& m, double p, double T,
-                                   double yl, TeamLDS<N>& L, int lane) {
+__device__ __forceinline__ double team_derivatives(const DeviceMech& m, double p,
+                                                   double T, double yl,
+                                                   TeamLDS<N>& L, int lane) {
     const int nS = m.nS;
     double rhoPart = 0.0, cpPart = 0.0, Ha = 0.0, cu = 0.0;
     if (lane < nS) {
@@ -118,40 +316,31 @@ __device__ __forceinline__ double team_derivatives(const DeviceMech& m, double p
     }
     const double rhoM = 1.0 / wave_sum(rhoPart);
     const double CpM = wave_sum(cpPart);
-    L.c[lane] = cu * rhoM;
+    const double cl = cu * rhoM;
+    L.c[lane] = cl;
     L.dN[lane] = 0.0;
+    third_body<N>(m, L, cl, lane);
+    // omega() evaluates the rates at T clipped to [Tlow, Thigh]
+    const TFun f = tfun(fmin(fmax(T, m.Tlow), m.Thigh));
+    specie_gibbs<N, false>(m, L, f, lane);
     __syncthreads();
-    const double Tc = fmin(fmax(T, m.Tlow), m.Thigh);
+#pragma unroll 1
     for (int pass = 0; pass < m.nPass; ++pass) {
         const int r = pass * kLanes + lane;
-        if (r >= m.nR) continue;
-        const int t = m.type[r];
-        const double* k = m.k + 3 * r;
-        double kf;
-        if (t == GFC_RATE_ARRHENIUS) {
-            kf = arrhenius(k, Tc);
-        } else {
-            const double M = third_body_M(m, pass, lane, L.c);
-            if (t == GFC_RATE_THIRD_BODY) {
-                kf = M * arrhenius(k, Tc);
-            } else {
-                const double k0 = arrhenius(k, Tc);
-                const double kInf = arrhenius(m.kInf + 3 * r, Tc);
-                const double Pr = k0 / kInf * M;
-                const double F = (t == GFC_RATE_TROE) ? troe_F(m.troe + 4 * r, Tc, Pr) : 1.0;
-                kf = kInf * (Pr / (1.0 + Pr)) * F;
+        if (r < m.nR) {
+            const Rxn x = load_rxn(m, pass, lane);
+            double kArr, ddT, pw[kTeamWidth];
+            const double kf = forward_rate(x, f, L.M, kArr, ddT);
+            const double kr = x.irr ? 0.0 : kf * exp(-ln_kc<N>(x, f, L, nullptr));
+            const double Cf = side_powers(x.li, x.lei, x.le, x.nl, L.c, pw);
+            const double Cr = side_powers(x.ri, x.rei, x.re, x.nr, L.c, pw);
+            const double w = kf * Cf - kr * Cr;
+#pragma unroll
+            for (int q = 0; q < kTeamWidth; ++q) {
+                if (q < x.nl) atomicAdd(&L.dN[x.li[q]], -(x.ls[q] * w));
+                if (q < x.nr) atomicAdd(&L.dN[x.ri[q]], x.rs[q] * w);
             }
         }
-        const double kr = m.irreversible[r] ? 0.0 : kf / reaction_Kc(m, r, Tc);
-        const long o = (long)r * kWidth;
-        const int nl = m.nl[r], nr = m.nr[r];
-        const double Cf = side_pow(m.li + o, m.lei + o, m.le + o, nl, L.c);
-        const double Cr = side_pow(m.ri + o, m.rei + o, m.re + o, nr, L.c);
-        const double w = kf * Cf - kr * Cr;
-#pragma unroll 1
-        for (int q = 0; q < nl; ++q) atomicAdd(&L.dN[m.li[o + q]], -(m.ls[o + q] * w));
-#pragma unroll 1
-        for (int q = 0; q < nr; ++q) atomicAdd(&L.dN[m.ri[o + q]], m.rs[o + q] * w);
     }
     __syncthreads();
     double d = 0.0;
@@ -166,8 +355,8 @@ __device__ __forceinline__ double team_derivatives(const DeviceMech& m, double p
 // pressure row/column dropped) in L.J and returns this lane's component of
 // the derivative computed alongside (the reference's "dfdx").
 template <int N>
-__device__ __forceinline__ double team_jacobian(const DeviceMech& m, double p, double T,
-                                double yl, TeamLDS<N>& L, int lane) {
+__device__ __noinline__ double team_jacobian(const DeviceMech& m, double p, double T,
+                                             double yl, TeamLDS<N>& L, int lane) {
     constexpr int P = TeamLDS<N>::P;
     const int nS = m.nS, n = nS + 1;
     double rhoPart = 0, cpPart = 0, dcpPart = 0, avPart = 0, Ha = 0, Cp = 0, cu = 0;
@@ -188,119 +377,148 @@ __device__ __forceinline__ double team_jacobian(const DeviceMech& m, double p, d
     const double CpM = wave_sum(cpPart);
     const double dCpMdT = wave_sum(dcpPart);
     const double alphavM = wave_sum(avPart) * rhoM;
-    L.c[lane] = cu * rhoM;
+    const double cl = cu * rhoM;
+    L.c[lane] = cl;
     L.dN[lane] = 0.0;
     L.Ha[lane] = Ha;
     if (lane < nS) {
         L.colA[lane] = rhoM / m.W[lane];
         L.colB[lane] = rhoM * (1.0 / (p / (m.R[lane] * T)));
     }
+    third_body<N>(m, L, cl, lane);
+    const TFun f = tfun(T);
+    specie_gibbs<N, true>(m, L, f, lane);
     for (int e = lane; e < n * P; e += kLanes) L.J[e] = 0.0;
-    __syncthreads();
+    const int nChunks = m.nSets > 0 ? (m.nSets + kMaxSets - 1) / kMaxSets : 1;
+    bool poisoned = false; // a fall-off rate with Cf - Cr/Kc not finite
 
-    for (int pass = 0; pass < m.nPass; ++pass) {
-        const int r = pass * kLanes + lane;
-        if (r >= m.nR) continue;
-        const int t = m.type[r];
-        const double* k = m.k + 3 * r;
-        const long o = (long)r * kWidth;
-        const int nl = m.nl[r], nr = m.nr[r];
-        double kf, ddT, ddcScale = 0.0;
-        if (t == GFC_RATE_ARRHENIUS) {
-            const double Ak = arrhenius(k, T);
-            kf = Ak;
-            ddT = Ak * (k[1] + k[2] / T) / T;
-        } else {
-            const double M = third_body_M(m, pass, lane, L.c);
-            if (t == GFC_RATE_THIRD_BODY) {
-                const double Ak = arrhenius(k, T);
-                const double dT = Ak * (k[1] + k[2] / T) / T;
-                kf = M * Ak;
-                ddT = M * dT;
-                ddcScale = Ak;
-            } else {
-                const double k0 = arrhenius(k, T);
-                const double* ki = m.kInf + 3 * r;
-                const double kInf = arrhenius(ki, T);
-                const double Pr = k0 / kInf * M;
-                const double F = (t == GFC_RATE_TROE) ? troe_F(m.troe + 4 * r, T, Pr) : 1.0;
-                const double temp = (Pr / (1.0 + Pr)) * F;
-                kf = kInf * temp;
-                ddT = (kInf * (ki[1] + ki[2] / T) / T) * temp;
+#pragma unroll 1
+    for (int chunk = 0; chunk < nChunks; ++chunk) {
+        for (int e = lane; e < kMaxSets * kLanes; e += kLanes) L.Wd[e] = 0.0;
+        __syncthreads();
+        const int s0 = chunk * kMaxSets;
+#pragma unroll 1
+        for (int pass = 0; pass < m.nPass; ++pass) {
+            const int r = pass * kLanes + lane;
+            if (r >= m.nR) continue;
+            const Rxn x = load_rxn(m, pass, lane);
+            const bool denseHere =
+                x.type == GFC_RATE_THIRD_BODY && x.set >= s0 && x.set < s0 + kMaxSets;
+            // sparse part in the first chunk; dense part with its set's chunk
+            if (chunk > 0 && !denseHere) continue;
+            double kArr, ddT, pwl[kTeamWidth], pwr[kTeamWidth];
+            const double kf = forward_rate(x, f, L.M, kArr, ddT);
+            const double Cf = side_powers(x.li, x.lei, x.le, x.nl, L.c, pwl);
+            double dwdT = Cf * ddT;
+            double w = Cf * kf;
+            double Cr = 0.0, Kc = 1.0, kr = 0.0;
+            if (!x.irr) {
+                double dKc;
+                const double lnKc = ln_kc<N>(x, f, L, &dKc);
+                Kc = exp(lnKc);
+                const double rKc = exp(-lnKc);
+                kr = kf * rKc;
+                Cr = side_powers(x.ri, x.rei, x.re, x.nr, L.c, pwr);
+                const double dkrdT = ddT * rKc - kr * dKc;
+                dwdT -= Cr * dkrdT;
+                w -= Cr * kr;
             }
-        }
-        const double Cf = side_pow(m.li + o, m.lei + o, m.le + o, nl, L.c);
-        double dwdT = Cf * ddT;
-        double w = Cf * kf;
-        double Cr = 0.0, Kc = 1.0, kr = 0.0;
-        const bool rev = !m.irreversible[r];
-        if (rev) {
-            double dKc;
-            reaction_Kc_dKc(m, r, T, Kc, dKc);
-            kr = kf / Kc;
-            Cr = side_pow(m.ri + o, m.rei + o, m.re + o, nr, L.c);
-            const double dkrdT = ddT / Kc - kr * dKc;
-            dwdT -= Cr * dkrdT;
-            w -= Cr * kr;
-        }
-        // dN/dt (the reference's dfdx)
-#pragma unroll 1
-        for (int i = 0; i < nl; ++i) atomicAdd(&L.dN[m.li[o + i]], -(m.ls[o + i] * w));
-#pragma unroll 1
-        for (int i = 0; i < nr; ++i) atomicAdd(&L.dN[m.ri[o + i]], m.rs[o + i] * w);
-        // jac_dCfdcj_contribution (gpuReaction.H:547-569)
-#pragma unroll 1
-        for (int j = 0; j < nl; ++j) {
-            const double dCf = side_dpow(m.li + o, m.lei + o, m.le + o, nl, j, L.c) * kf;
-            const int sj = m.li[o + j];
-#pragma unroll 1
-            for (int i = 0; i < nl; ++i)
-                atomicAdd(&L.J[m.li[o + i] * P + sj], -(m.ls[o + i] * dCf));
-#pragma unroll 1
-            for (int i = 0; i < nr; ++i)
-                atomicAdd(&L.J[m.ri[o + i] * P + sj], m.rs[o + i] * dCf);
-        }
-        // jac_dCrdcj_contribution (:571-593)
-        if (rev) {
-#pragma unroll 1
-            for (int j = 0; j < nr; ++j) {
-                const double dCr = side_dpow(m.ri + o, m.rei + o, m.re + o, nr, j, L.c) * kr;
-                const int sj = m.ri[o + j];
-#pragma unroll 1
-                for (int i = 0; i < nl; ++i)
-                    atomicAdd(&L.J[m.li[o + i] * P + sj], m.ls[o + i] * dCr);
-#pragma unroll 1
-                for (int i = 0; i < nr; ++i)
-                    atomicAdd(&L.J[m.ri[o + i] * P + sj], -(m.rs[o + i] * dCr));
-            }
-        }
-        // jac_dCdT_contribution (:595-612)
-#pragma unroll 1
-        for (int i = 0; i < nl; ++i) atomicAdd(&L.J[m.li[o + i] * P + nS], -(m.ls[o + i] * dwdT));
-#pragma unroll 1
-        for (int i = 0; i < nr; ++i) atomicAdd(&L.J[m.ri[o + i] * P + nS], m.rs[o + i] * dwdT);
-        // dense concentration derivative of third-body rates
-        if (t != GFC_RATE_ARRHENIUS) {
             const double temp = (Cf - Cr / Kc);
-            const double* e = m.effP + (long)pass * nS * kLanes + lane;
-            if (t == GFC_RATE_THIRD_BODY || !isfinite(temp)) {
-                const double scale = (t == GFC_RATE_THIRD_BODY) ? ddcScale : 0.0;
-#pragma unroll 1
-                for (int j = 0; j < nS; ++j) {
-                    const double ddc = (t == GFC_RATE_THIRD_BODY)
-                                           ? (e[(long)j * kLanes] * scale) * temp
-                                           : 0.0 * temp;
-#pragma unroll 1
-                    for (int i = 0; i < nl; ++i)
-                        atomicAdd(&L.J[m.li[o + i] * P + j], -(m.ls[o + i] * ddc));
-#pragma unroll 1
-                    for (int i = 0; i < nr; ++i)
-                        atomicAdd(&L.J[m.ri[o + i] * P + j], m.rs[o + i] * ddc);
+            if (denseHere) {
+                // ddc_j = eff_j k (Cf - Cr/Kc): row weights of the set
+                const double g = kArr * temp;
+                double* Wd = &L.Wd[(x.set - s0) * kLanes];
+#pragma unroll
+                for (int i = 0; i < kTeamWidth; ++i) {
+                    if (i < x.nl) atomicAdd(&Wd[x.li[i]], -(x.ls[i] * g));
+                    if (i < x.nr) atomicAdd(&Wd[x.ri[i]], x.rs[i] * g);
+                }
+            }
+            if (chunk > 0) continue;
+            if (x.type > GFC_RATE_THIRD_BODY && !isfinite(temp)) poisoned = true;
+            // dN/dt (the reference's dfdx) and dCdT (gpuReaction.H:595-612)
+#pragma unroll
+            for (int i = 0; i < kTeamWidth; ++i) {
+                if (i < x.nl) {
+                    atomicAdd(&L.dN[x.li[i]], -(x.ls[i] * w));
+                    atomicAdd(&L.J[x.li[i] * P + nS], -(x.ls[i] * dwdT));
+                }
+                if (i < x.nr) {
+                    atomicAdd(&L.dN[x.ri[i]], x.rs[i] * w);
+                    atomicAdd(&L.J[x.ri[i] * P + nS], x.rs[i] * dwdT);
+                }
+            }
+            // jac_dCfdcj_contribution (gpuReaction.H:547-569)
+#pragma unroll
+            for (int j = 0; j < kTeamWidth; ++j) {
+                if (j >= x.nl) break;
+                const double dCf = side_dpow(x.li, x.lei, x.le, x.nl, j, L.c, pwl) * kf;
+                const int sj = x.li[j];
+#pragma unroll
+                for (int i = 0; i < kTeamWidth; ++i) {
+                    if (i < x.nl) atomicAdd(&L.J[x.li[i] * P + sj], -(x.ls[i] * dCf));
+                    if (i < x.nr) atomicAdd(&L.J[x.ri[i] * P + sj], x.rs[i] * dCf);
+                }
+            }
+            // jac_dCrdcj_contribution (:571-593)
+            if (!x.irr) {
+#pragma unroll
+                for (int j = 0; j < kTeamWidth; ++j) {
+                    if (j >= x.nr) break;
+                    const double dCr = side_dpow(x.ri, x.rei, x.re, x.nr, j, L.c, pwr) * kr;
+                    const int sj = x.ri[j];
+#pragma unroll
+                    for (int i = 0; i < kTeamWidth; ++i) {
+                        if (i < x.nl) atomicAdd(&L.J[x.li[i] * P + sj], x.ls[i] * dCr);
+                        if (i < x.nr) atomicAdd(&L.J[x.ri[i] * P + sj], -(x.rs[i] * dCr));
+                    }
                 }
             }
         }
+        __syncthreads();
+        // dense third-body part, column-parallel: J(i, j) += sum_s Wd[s][i] eff_s[j]
+        if (lane < nS) {
+            const int ns = min(kMaxSets, m.nSets - s0);
+            double e[kMaxSets];
+#pragma unroll
+            for (int s = 0; s < kMaxSets; ++s)
+                e[s] = (s < ns) ? m.effSets[(s0 + s) * kLanes + lane] : 0.0;
+#pragma unroll 2
+            for (int i = 0; i < nS; ++i) {
+                double acc = 0.0;
+#pragma unroll
+                for (int s = 0; s < kMaxSets; ++s)
+                    if (s < ns) acc += L.Wd[s * kLanes + i] * e[s];
+                L.J[i * P + lane] += acc;
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
+    // a fall-off reaction with a non-finite Cf - Cr/Kc makes the reference's
+    // dense (zero) concentration derivative NaN for its species rows
+    if (__any(poisoned)) {
+        const double qnan = __longlong_as_double(0x7ff8000000000000LL);
+#pragma unroll 1
+        for (int pass = 0; pass < m.nPass; ++pass) {
+            const int r = pass * kLanes + lane;
+            if (r < m.nR) {
+                const Rxn x = load_rxn(m, pass, lane);
+                if (x.type > GFC_RATE_THIRD_BODY) {
+                    double pw[kTeamWidth];
+                    const double Cf = side_powers(x.li, x.lei, x.le, x.nl, L.c, pw);
+                    const double Cr = side_powers(x.ri, x.rei, x.re, x.nr, L.c, pw);
+                    const double temp = Cf - Cr / exp(ln_kc<N>(x, f, L, nullptr));
+                    if (!isfinite(temp)) {
+                        for (int i = 0; i < x.nl; ++i)
+                            for (int j = 0; j < nS; ++j) L.J[x.li[i] * P + j] = qnan;
+                        for (int i = 0; i < x.nr; ++i)
+                            for (int j = 0; j < nS; ++j) L.J[x.ri[i] * P + j] = qnan;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
     // derivative: dY/dt and dT/dt
     double d = 0.0;
     if (lane < nS) d = L.dN[lane] * (m.W[lane] / rhoM);
@@ -310,6 +528,7 @@ __device__ __forceinline__ double team_jacobian(const DeviceMech& m, double p, d
         double* row = &L.J[lane * P];
         const double WiByrhoM = m.W[lane] / rhoM;
         double ddNdT = row[nS];
+#pragma unroll 4
         for (int j = 0; j < nS; ++j) {
             const double tj = row[j];
             ddNdT -= tj * L.c[j] * alphavM;
@@ -321,6 +540,7 @@ __device__ __forceinline__ double team_jacobian(const DeviceMech& m, double p, d
     // ddT_dtdY (:108-126): lane i sums column i
     if (lane < nS) {
         double v = 0.0;
+#pragma unroll 4
         for (int j = 0; j < nS; ++j) v -= L.J[j * P + lane] * L.Ha[j];
         v -= Cp * dfdxT;
         L.J[nS * P + lane] = v / CpM;
@@ -335,47 +555,38 @@ __device__ __forceinline__ double team_jacobian(const DeviceMech& m, double p, d
 }
 
 // --------------------------------------------------------------- LU ----
-// Factorises the rows held in registers (row-per-lane), writing the pivot
-// lane of each step to L.pl.  `ps` receives the step at which this lane
-// was the pivot and `rd` the reciprocal of its pivot.
+// Row-per-lane factorisation.  On return `ps` is the step at which this
+// lane's row was the pivot, `rd` the reciprocal of its pivot and lane j's
+// `plv` the pivot lane of step j.
 template <int N>
-__device__ __forceinline__ void team_lu(double (&row)[N], int n, int lane,
-                                        TeamLDS<N>& L, int& ps, double& rd) {
+__device__ __forceinline__ void team_lu(double (&row)[N], int n, int lane, int& ps,
+                                        double& rd, int& plv) {
     double amax = 0.0;
 #pragma unroll
     for (int k = 0; k < N; ++k) amax = fmax(amax, fabs(row[k]));
-    // the reference keeps the first strict maximum; the maximum value is
-    // the same either way
     const double vv = 1.0 / amax;
     int pos = lane;
-    bool done = lane >= n;
+    bool done = lane >= N;
     ps = lane;
     rd = 1.0;
+    plv = lane;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        if (j >= n) continue;
-        double key = done ? -1.0 : vv * fabs(row[j]);
-        if (!(key >= 0.0)) key = -1.0; // NaN never wins (the reference's >=)
-        int tag = (pos << 8) | lane;
-#pragma unroll
-        for (int s = 32; s >= 1; s >>= 1) {
-            const double ok = __shfl_xor(key, s);
-            const int ot = __shfl_xor(tag, s);
-            if (ok > key || (ok == key && ot > tag)) {
-                key = ok;
-                tag = ot;
-            }
-        }
-        const int t = __builtin_amdgcn_readfirstlane(tag);
-        const int pv = t & 0xff;      // pivot lane
-        const int P = t >> 8;         // its position (the reference's iMax)
+        // candidates: rows not yet pivots (padding rows only once the real
+        // columns are done); NaN keys never win (the reference's >=)
+        double key = (done || (lane >= n && j < n)) ? -1.0 : vv * fabs(row[j]);
+        if (!(key >= 0.0)) key = -1.0;
+        const double kmax = wave_max(key);
+        const int P = wave_max_i((key == kmax && !done) ? pos : -1);
+        const int pv = __builtin_amdgcn_readfirstlane(
+            (int)__builtin_ctzll(__ballot(pos == P && !done)));
+        if (lane == j) plv = pv;
         if (pos == j) pos = P;
         if (lane == pv) {
             pos = j;
             done = true;
             ps = j;
         }
-        if (lane == 0) L.pl[j] = pv;
         double dj = readlane(row[j], pv);
         if (dj == 0.0) {
             dj = kSmall;
@@ -383,38 +594,37 @@ __device__ __forceinline__ void team_lu(double (&row)[N], int n, int lane,
         }
         const double rDiag = 1.0 / dj;
         if (lane == pv) rd = rDiag;
-        if (!done) {
-            const double l = row[j] * rDiag;
-            row[j] = l;
+        const double l = done ? 0.0 : row[j] * rDiag;
+        if (!done) row[j] = l;
+        // update in groups of 8 columns; the scheduling barrier keeps the
+        // broadcast values of one group in flight at a time
 #pragma unroll
-            for (int k = j + 1; k < N; ++k) {
-                if (k < n) row[k] -= l * readlane(row[k], pv);
-            }
+        for (int k = j + 1; k < N; ++k) {
+            row[k] = fma(-l, readlane(row[k], pv), row[k]);
+            if ((k - j) % 8 == 0) __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
 
-// Solves (LU) x = b for the right-hand side held species-parallel (lane i:
-// component i); returns x species-parallel.
+// Solves (LU) x = b for b held species-parallel; returns x species-parallel.
 template <int N>
-__device__ __forceinline__ double team_solve(const double (&row)[N], int n,
-                                             int lane, const TeamLDS<N>& L,
-                                             int ps, double rd, double b) {
-    // forward: lane p_j's b is final at step j
+__device__ __forceinline__ double team_solve(const double (&row)[N], int lane, int ps,
+                                             double rd, int plv, double b) {
+    // keep the per-step pivot lanes from being hoisted (and held in scalar
+    // registers) across the calls of one attempt
+    asm volatile("" : "+v"(plv));
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        if (j >= n) continue;
-        const int pv = __builtin_amdgcn_readfirstlane(L.pl[j]);
+        const int pv = __builtin_amdgcn_readlane(plv, j);
         const double yj = readlane(b, pv);
-        if (ps > j) b -= row[j] * yj;
+        if (ps > j) b = fma(-row[j], yj, b);
     }
     double x = 0.0;
 #pragma unroll
     for (int j = N - 1; j >= 0; --j) {
-        if (j >= n) continue;
-        const int pv = __builtin_amdgcn_readfirstlane(L.pl[j]);
+        const int pv = __builtin_amdgcn_readlane(plv, j);
         const double xj = readlane(b * rd, pv);
-        if (ps < j) b -= row[j] * xj;
+        if (ps < j) b = fma(-row[j], xj, b);
         if (lane == j) x = xj;
     }
     return x;
@@ -448,8 +658,7 @@ team_solve_kernel(SolveArgs A, int* queue) {
         const double p = A.Y0[(long)(nS + 1) * ld + cell];
         double dtChem = A.dtChem[cell];
         int status = 0, attempts = 0, accepted = 0;
-        double T = uniform(readlane(y, nS));
-        if (T > A.Treact) {
+        if (readlane(y, nS) > A.Treact) {
             double timeLeft = A.deltaT;
             while (timeLeft > kSmall) {
                 const double xEnd = timeLeft;
@@ -465,7 +674,7 @@ team_solve_kernel(SolveArgs A, int* queue) {
                     }
                     // adaptiveSolve (gpuAdaptiveSolver.H:83-134)
                     double dx = stepDx;
-                    T = readlane(y, nS);
+                    const double T = readlane(y, nS);
                     const double dydx0 = team_derivatives<N>(m, p, T, y, L, lane);
                     const double dfdx = team_jacobian<N>(m, p, T, y, L, lane);
                     double err, yNew;
@@ -480,9 +689,9 @@ team_solve_kernel(SolveArgs A, int* queue) {
                             if (k == lane) v += (lane < n) ? tdiag : 1.0;
                             row[k] = v;
                         }
-                        int ps;
+                        int ps, plv;
                         double rd;
-                        team_lu<N>(row, n, lane, L, ps, rd);
+                        team_lu<N>(row, n, lane, ps, rd, plv);
                         // Rosenbrock stages (gpuRosenbrock{12,23,34}.H::solve)
                         double k1 = 0, k2 = 0, k3 = 0, k4 = 0, dydx = 0;
 #pragma unroll 1
@@ -497,7 +706,7 @@ team_solve_kernel(SolveArgs A, int* queue) {
                             else if (st == 1) rhs = dydx + dx * tb.d2 * dfdx + tb.c21 * k1 / dx;
                             else if (st == 2) rhs = dydx + dx * tb.d3 * dfdx + (tb.c31 * k1 + tb.c32 * k2) / dx;
                             else rhs = dydx + dx * tb.d4 * dfdx + (tb.c41 * k1 + tb.c42 * k2 + tb.c43 * k3) / dx;
-                            const double kx = team_solve<N>(row, n, lane, L, ps, rd, rhs);
+                            const double kx = team_solve<N>(row, lane, ps, rd, plv, rhs);
                             if (st == 0) k1 = kx;
                             else if (st == 1) k2 = kx;
                             else if (st == 2) k3 = kx;
@@ -519,7 +728,7 @@ team_solve_kernel(SolveArgs A, int* queue) {
                             const double tol = A.absTol + A.relTol * fmax(fabs(y), fabs(yNew));
                             ne = fabs(e) / tol;
                         }
-                        err = uniform(wave_max(ne));
+                        err = wave_max(ne);
                         ++attempts;
                         if (err > 1) {
                             const double scale = fmax(A.safeScale * pow(err, -A.alphaDec), A.minScale);
@@ -579,8 +788,9 @@ team_solve_kernel(SolveArgs A, int* queue) {
     }
 }
 
-template __global__ void team_solve_kernel<16>(SolveArgs, int*);
-template __global__ void team_solve_kernel<56>(SolveArgs, int*);
+template __global__ void team_solve_kernel<11>(SolveArgs, int*);
+template __global__ void team_solve_kernel<32>(SolveArgs, int*);
+template __global__ void team_solve_kernel<54>(SolveArgs, int*);
 template __global__ void team_solve_kernel<64>(SolveArgs, int*);
 
 } // namespace gfc

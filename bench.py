@@ -1,0 +1,184 @@
+"""Benchmark: chemistry cells per second for one CFD time step on one MI355X.
+
+Workload (default): the 53-specie / 325-reaction GRI methane-air mechanism,
+262144 cells in FP64, Rosenbrock34 with relTol 1e-6 and absTol 1e-10, one
+CFD step deltaT = 1e-6 s at p = 1e5 Pa; half unburnt CH4-air at
+300-900 K, half near-equilibrium products at 1800-2400 K, mass fractions
+jittered by +-5% (synthetic, seeded).  A timed step is one full chemistry
+solve of all cells (the batched persistent kernel replayed as a hipGraph),
+with the chemical time step reset to its initial value before each step so
+every step does the same work.  Inputs are resident in HBM.
+
+Printed (one JSON line): the headline metric and, beside it, the plain
+per-cell kernel (one lane per cell, the reference's mapping) on a sample of
+the same states, the FP64 roofline fraction of the batched kernel, and the
+CPU baseline: gpuFoam's own per-cell code built for the host (or the oracle
+restatement where that build is absent) on the host cores, on a bounded
+sample of the same states.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--config gri_production]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+
+
+def cpu_baseline(config: str, sample: int, threads: int) -> dict:
+    """gpuFoam's per-cell solver built for the host (oracle/_ref) or the
+    oracle, timed on `sample` cells of the same state law."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    from gpufoam_amd import ode_inputs
+    from gpufoam_amd.workloads import make_states
+    import oracle_lib
+    m, Y0, dt0, c = make_states(config, sample)
+    ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
+    if oracle_lib.ref_available():
+        runner, kind = oracle_lib.Reference(m), "reference"
+    else:
+        runner, kind = oracle_lib.Oracle(m), "port"
+    runner.solve_cells(ode, Y0[:, :64], c["deltaT"], dt0[:64], threads=threads)  # warm
+    t = time.perf_counter()
+    runner.solve_cells(ode, Y0, c["deltaT"], dt0, threads=threads)
+    el = time.perf_counter() - t
+    return {"value": sample / el, "unit": "cells/s", "cores": threads, "kind": kind,
+            "sample": f"{sample} cells of the same {config} state law, one CFD step "
+                      f"({el:.1f} s wall on {threads} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="gri_production")
+    ap.add_argument("--cells", type=int, default=0, help="override the cell count")
+    ap.add_argument("--plain-sample", type=int, default=16384)
+    ap.add_argument("--cpu-sample", type=int, default=24576)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the chemistry of each cell is "
+              "independent, so N GPUs would each run this same kernel on their "
+              "share of cells (see DESIGN.md); no multi-GPU figure is produced.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from gpufoam_amd import ode_inputs
+    from gpufoam_amd.evaluator import Evaluator
+    from gpufoam_amd.workloads import CONFIGS, make_states
+    from gpufoam_amd.workmodel import attempt_flops, work_flops
+
+    cfg = CONFIGS[args.config]
+    n = args.cells or cfg["cells"]
+    m, Y0, dt0, c = make_states(args.config, n)
+    ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
+    dev = torch.device("cuda:0")
+    dY0 = torch.tensor(Y0, device=dev)
+    dYn = torch.empty_like(dY0)
+    ddt0 = torch.tensor(dt0, device=dev)
+    ddt = ddt0.clone()
+    dmin = torch.zeros(1, dtype=torch.float64, device=dev)
+    stats = torch.zeros((3, n), dtype=torch.int32, device=dev)
+    ev = Evaluator(m, ode, 0.0, max_cells=n)
+    graph = not args.no_graph
+
+    def step(with_stats=False):
+        ddt.copy_(ddt0)
+        ev.solve(c["deltaT"], dY0, ddt, Ynew=dYn, deltaTMin=dmin,
+                 stats=stats if with_stats else None, batched=True, graph=graph)
+
+    # first call with statistics (work accounting), then warmup
+    step(with_stats=True)
+    torch.cuda.synchronize()
+    st = stats.cpu().numpy()
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = e0.elapsed_time(e1) / args.steps
+    value = n / (ms_per_step / 1e3)
+
+    stages = {12: 2, 23: 3, 34: 4}[ode.solver]
+    flops = work_flops(m, st[1], st[2], stages)
+    achieved = flops / (ms_per_step / 1e3) / 1e12
+
+    # plain per-cell kernel on a sample of the same states
+    ns = min(args.plain_sample, n)
+    evp = Evaluator(m, ode, 0.0, max_cells=ns)
+    dY0p = dY0[:, :ns].contiguous()
+    dYnp = torch.empty_like(dY0p)
+    ddtp = ddt0[:ns].clone()
+    evp.solve(c["deltaT"], dY0p, ddtp, Ynew=dYnp, ldY=ns)  # warm
+    torch.cuda.synchronize()
+    ddtp.copy_(ddt0[:ns])
+    p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    p0.record(stream)
+    evp.solve(c["deltaT"], dY0p, ddtp, Ynew=dYnp, ldY=ns)
+    p1.record(stream)
+    torch.cuda.synchronize()
+    plain_value = ns / (p0.elapsed_time(p1) / 1e3)
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cpu = None if args.no_cpu else cpu_baseline(args.config, min(args.cpu_sample, n), threads)
+
+    traffic = None
+    tfile = os.path.join(HERE, "profiles", f"hbm_traffic_{args.config}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    out = {
+        "metric": "chem cells/s per CFD dt, 53sp FP64 rtol1e-6 (batched persistent kernel)",
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.config, "mechanism": cfg["mech"],
+                   "species": m.n_species, "reactions": m.n_reactions, "cells": n,
+                   "solver": c["solver"], "relTol": c["relTol"], "absTol": c["absTol"],
+                   "deltaT": c["deltaT"], "graph": graph},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": traffic,
+                     "work_model_flops_per_attempt": attempt_flops(m, stages)["attempt"]},
+        "cpu_baseline": cpu,
+        "plain_cells_per_s": plain_value,
+        "plain_sample_cells": ns,
+        "batched_over_plain": value / plain_value,
+        "attempts_per_cell": float(st[1].mean()),
+        "accepted_per_cell": float(st[2].mean()),
+        "wall_s": wall,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

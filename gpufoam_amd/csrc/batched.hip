@@ -4,10 +4,11 @@
 //    wavefronts the device can hold resident; every wavefront takes the
 //    next cell from one atomic counter until the cells run out, so cells
 //    with hundreds of sub-steps do not hold up a statically assigned tile.
-//  * hipGraph: with GFC_GRAPH the queue reset, the persistent kernel and
-//    the deltaTMin epilogue are captured once and the executable graph is
-//    replayed on later calls with identical arguments (one CFD time step
-//    = one graph launch); changed arguments update the graph in place.
+//  * hipGraph: with GFC_GRAPH the queue reset and the persistent kernel
+//    are recorded once (on a private stream) and the executable graph is
+//    replayed on the caller's stream on later calls with identical
+//    arguments; changed arguments update the graph in place.  Called inside
+//    the caller's own stream capture, the launches join that graph instead.
 //  * Launch shape (autotuned by gpufoam_amd.autotune): cells per wavefront
 //    1 (the wavefront kernel) or 64 (one lane per cell), and the number of
 //    wavefronts kept resident per compute unit.
@@ -33,6 +34,7 @@ struct BatchedState {
     int numCUs = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipStream_t captureStream = nullptr;  // private stream the graph is recorded on
     SolveArgs lastArgs{};
     int lastGrid = 0;
     int lastN = 0;
@@ -68,6 +70,11 @@ int batched_init(gfc_evaluator* ev) {
     if (hipFail(hipGetDeviceProperties(&prop, ev->device), "hipGetDeviceProperties"))
         return GFC_ERR_HIP;
     b->numCUs = prop.multiProcessorCount;
+    // Graphs are recorded on a private stream and launched on the caller's:
+    // the caller's stream may be the null stream, which cannot be captured.
+    if (hipFail(hipStreamCreateWithFlags(&b->captureStream, hipStreamNonBlocking),
+                "hipStreamCreateWithFlags"))
+        return GFC_ERR_HIP;
     return GFC_OK;
 }
 
@@ -77,6 +84,7 @@ void batched_free(gfc_evaluator* ev) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->queue) (void)hipFree(b->queue);
+    if (b->captureStream) (void)hipStreamDestroy(b->captureStream);
     delete b;
     ev->batched = nullptr;
 }
@@ -125,6 +133,13 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
     if (ev->wavesPerBlock > 0 && ev->wavesPerBlock < perCU) perCU = ev->wavesPerBlock;
     int grid = b->numCUs * perCU;
     if (grid > A.nCells) grid = A.nCells > 0 ? A.nCells : 1;
+    if (graph) {
+        // inside a caller's own capture the launches simply join that graph
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (s && hipStreamIsCapturing(s, &cs) == hipSuccess &&
+            cs != hipStreamCaptureStatusNone)
+            graph = false;
+    }
     if (!graph) return enqueue(ev, b, At, s, fn, grid);
 
     // graph path: reuse the executable graph when nothing changed
@@ -132,11 +147,12 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
                       std::memcmp(&b->lastArgs, &At, sizeof(SolveArgs)) == 0;
     if (!same) {
         hipGraph_t g = nullptr;
-        if (hipFail(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal),
+        hipStream_t cs = b->captureStream;
+        if (hipFail(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal),
                     "hipStreamBeginCapture"))
             return GFC_ERR_HIP;
-        int st = enqueue(ev, b, At, s, fn, grid);
-        hipError_t e = hipStreamEndCapture(s, &g);
+        int st = enqueue(ev, b, At, cs, fn, grid);
+        hipError_t e = hipStreamEndCapture(cs, &g);
         if (st != GFC_OK) return st;
         if (hipFail(e, "hipStreamEndCapture")) return GFC_ERR_HIP;
         bool updated = false;

@@ -312,8 +312,9 @@ int gfc_solve(gfc_evaluator* ev, double deltaT, double deltaTChemMax,
               int nCells, int ldY, const double* rho, const double* Y0,
               double* Ynew, double* RR, double* deltaTChem, double* deltaTMin,
               int* cellStats, void* stream, unsigned flags) {
-    if (!ev || !Y0 || !deltaTChem || nCells < 0 || ldY < nCells ||
-        !(deltaT > 0) || (RR && !rho)) {
+    // with no cells the field pointers may be null (zero-size buffers)
+    if (!ev || nCells < 0 || ldY < nCells || !(deltaT > 0) ||
+        (nCells > 0 && (!Y0 || !deltaTChem || (RR && !rho)))) {
         setLastError("gfc_solve: bad argument");
         return GFC_ERR_INVALID_ARGUMENT;
     }

@@ -1,0 +1,238 @@
+"""Parity of the HIP kernels with the CPU oracle (MI355X).
+
+Both device forms -- the plain per-cell kernel (one lane per cell, the
+reference's mapping) and the batched persistent wavefront-per-cell kernel --
+are compared with the oracle on the same seeded states:
+
+  species  |Y_gpu - Y_cpu| <= 1e-6 |Y_cpu| + 1e-10      (rtol 1e-6, absTol)
+  T        |T_gpu - T_cpu| <= 1e-8 T_cpu
+  RR       from the same Ynew, with rho
+  deltaTChem and the step counts: equal wherever the accept/reject sequence
+  is the same (ulp-level differences of exp/log and of summation order can
+  move a step by one attempt; such cells are reported and bounded)
+
+The oracle is bit-identical to gpuFoam's own code built for the host
+(tests/test_oracle.py)."""
+import numpy as np
+import pytest
+
+from gpufoam_amd import GfcError, ode_inputs
+from gpufoam_amd.workloads import make_states
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+RTOL_Y, ATOL_Y, RTOL_T = 1e-6, 1e-10, 1e-8
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from gpufoam_amd import _capi
+    _capi.lib()  # the HIP library must be present: no fallback
+
+
+def _oracle_run(cfg, n, **kw):
+    from oracle_lib import Oracle
+    m, Y0, dt0, c = make_states(cfg, n)
+    ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
+    rho = np.linspace(0.3, 1.3, n)
+    ref = Oracle(m).solve_cells(ode, Y0, c["deltaT"], dt0, rho=rho, want_rr=True,
+                                deltaTChemMax=kw.get("deltaTChemMax", 1e300))
+    return m, Y0, dt0, c, ode, rho, ref
+
+
+def _device_run(m, Y0, dt0, c, ode, rho, batched, graph=False, deltaTChemMax=1e300,
+                Treact=0.0):
+    from gpufoam_amd.evaluator import Evaluator
+    n = Y0.shape[1]
+    ev = Evaluator(m, ode, Treact, max_cells=n)
+    dY0 = torch.tensor(Y0, device="cuda")
+    dYn = torch.empty_like(dY0)
+    dRR = torch.zeros((m.n_species, n), dtype=torch.float64, device="cuda")
+    ddt = torch.tensor(dt0, device="cuda")
+    dmin = torch.zeros(1, dtype=torch.float64, device="cuda")
+    st = torch.zeros((3, n), dtype=torch.int32, device="cuda")
+    drho = torch.tensor(rho, device="cuda")
+    ev.solve(c["deltaT"], dY0, ddt, Ynew=dYn, RR=dRR, rho=drho, deltaTMin=dmin, stats=st,
+             batched=batched, graph=graph, deltaTChemMax=deltaTChemMax)
+    torch.cuda.synchronize()
+    return (dYn.cpu().numpy(), ddt.cpu().numpy(), dmin.item(), st.cpu().numpy(),
+            dRR.cpu().numpy())
+
+
+def _check(m, got, ref, tight_dt, Y0=None, rho=None, deltaT=None):
+    Yg, dtg, dming, stg, rrg = got
+    Yo, dto, dmino, sto, rro = ref
+    nS = m.n_species
+    errY = np.abs(Yg[:nS] - Yo[:nS]) / (RTOL_Y * np.abs(Yo[:nS]) + ATOL_Y)
+    errT = np.abs(Yg[nS] - Yo[nS]) / Yo[nS]
+    print(f"species error / tolerance max {errY.max():.3g}; T rel max {errT.max():.3g}")
+    assert errY.max() <= 1.0
+    assert errT.max() <= RTOL_T
+    assert np.array_equal(Yg[nS + 1], Yo[nS + 1])  # pressure untouched
+    assert (stg[0] == sto[0]).all()                 # status bits
+    same = stg[1] == sto[1]
+    frac = same.mean()
+    print(f"identical step sequence in {frac:.3f} of cells")
+    assert frac >= (1.0 if tight_dt else 0.8)
+    assert np.allclose(dtg[same], dto[same], rtol=1e-6, atol=0)
+    if tight_dt:
+        assert np.isclose(dming, dmino, rtol=1e-6)
+    if Y0 is not None:
+        # RR = rho (Ynew - Y0) / deltaT of the kernel's own Ynew
+        # (gpuKernelEvaluator.cu:84-89), and within the state tolerance of
+        # the oracle's RR
+        want = rho * (Yg[:nS] - Y0[:nS]) / deltaT
+        assert np.allclose(rrg, want, rtol=1e-12, atol=1e-300)
+        bound = rho * (RTOL_Y * np.abs(Yo[:nS]) + ATOL_Y) / deltaT
+        assert np.all(np.abs(rrg - rro) <= 1.01 * bound)
+
+
+CASES = [("h2_small", 384), ("gri_production", 192), ("gri_batching", 32), ("h2_adjoint", 48)]
+
+
+@pytest.mark.parametrize("cfg,n", CASES)
+def test_plain_kernel_matches_oracle(cfg, n):
+    print(f"case {cfg} n={n} plain", flush=True)
+    m, Y0, dt0, c, ode, rho, ref = _oracle_run(cfg, n)
+    got = _device_run(m, Y0, dt0, c, ode, rho, batched=False)
+    _check(m, got, ref, tight_dt=True, Y0=Y0, rho=rho, deltaT=c["deltaT"])
+
+
+@pytest.mark.parametrize("cfg,n", CASES)
+def test_batched_kernel_matches_oracle(cfg, n):
+    print(f"case {cfg} n={n} batched", flush=True)
+    m, Y0, dt0, c, ode, rho, ref = _oracle_run(cfg, n)
+    got = _device_run(m, Y0, dt0, c, ode, rho, batched=True)
+    _check(m, got, ref, tight_dt=False, Y0=Y0, rho=rho, deltaT=c["deltaT"])
+
+
+@pytest.mark.parametrize("solver", ["Rosenbrock12", "Rosenbrock23"])
+@pytest.mark.parametrize("batched", [False, True])
+def test_other_solvers(solver, batched):
+    """Ros12 and Ros23 on H2 states with the reference tutorial tolerances
+    (tutorials/h2: absTol 1e-12, relTol 0.1 -- here relTol 1e-5)."""
+    from oracle_lib import Oracle
+    m, Y0, dt0, c = make_states("h2_small", 128)
+    ode = ode_inputs(solver, relTol=1e-5, absTol=1e-10)
+    rho = np.ones(128)
+    ref = Oracle(m).solve_cells(ode, Y0, 1e-5, dt0, rho=rho, want_rr=True)
+    got = _device_run(m, Y0, dt0, c | {"deltaT": 1e-5}, ode, rho, batched=batched)
+    _check(m, got, ref, tight_dt=not batched)
+
+
+def test_batched_is_deterministic_and_graph_replays():
+    m, Y0, dt0, c = make_states("gri_production", 96)
+    ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
+    rho = np.ones(96)
+    a = _device_run(m, Y0, dt0, c, ode, rho, batched=True)
+    b = _device_run(m, Y0, dt0, c, ode, rho, batched=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    from gpufoam_amd.evaluator import Evaluator
+    ev = Evaluator(m, ode, 0.0, max_cells=96)
+    dY0 = torch.tensor(Y0, device="cuda")
+    outs = []
+    for rep in range(3):  # capture, then two replays, with host work between
+        dYn = torch.empty_like(dY0)
+        ddt = torch.tensor(dt0, device="cuda")
+        ev.solve(c["deltaT"], dY0, ddt, Ynew=dYn, batched=True, graph=True)
+        torch.cuda.synchronize()
+        outs.append((dYn.cpu().numpy(), ddt.cpu().numpy()))
+        _ = np.linalg.norm(outs[-1][0])
+    for o in outs:
+        assert np.array_equal(o[0], a[0]) and np.array_equal(o[1], a[1])
+
+
+def test_plain_lane_shape_equals_plain():
+    """The batched launcher with one lane per cell runs the plain kernel."""
+    from gpufoam_amd.evaluator import Evaluator
+    m, Y0, dt0, c = make_states("h2_small", 100)
+    ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
+    a = _device_run(m, Y0, dt0, c, ode, np.ones(100), batched=False)
+    ev = Evaluator(m, ode, 0.0, max_cells=100)
+    ev.set_batched_shape(64)
+    Yn, dt, dmin, st, _ = ev.solve_host(c["deltaT"], Y0, dt0, batched=True)
+    assert np.array_equal(Yn, a[0]) and np.array_equal(dt, a[1])
+
+
+def test_treact_gate_and_clamp():
+    """Cells at or below Treact are not integrated and get deltaTChem =
+    deltaT (singleCellSolver.H:31,48-50); deltaTChem is clamped to
+    deltaTChemMax after deltaTMin is taken (gpuKernelEvaluator.cu:91-94)."""
+    from gpufoam_amd.evaluator import Evaluator
+    m, Y0, dt0, c = make_states("h2_small", 64)
+    nS = m.n_species
+    Treact = 1200.0
+    ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
+    for batched in (False, True):
+        ev = Evaluator(m, ode, Treact, max_cells=64)
+        Yn, dt, dmin, st, rr = ev.solve_host(c["deltaT"], Y0, dt0, deltaTChemMax=2e-6,
+                                             want_rr=True, batched=batched)
+        cold = Y0[nS] <= Treact
+        assert cold.any() and (~cold).any()
+        assert np.array_equal(Yn[:, cold], Y0[:, cold])
+        assert (st[0][cold] == 4).all() and (st[1][cold] == 0).all()
+        assert (rr[:, cold] == 0).all()
+        assert np.all(dt <= 2e-6)
+        assert np.isclose(dmin, min(c["deltaT"], dmin))
+
+
+def test_host_pointer_api_matches_reference_interface():
+    """computeRR / computeYNew keep the reference's cell-major layout."""
+    from gpufoam_amd.evaluator import Evaluator
+    from oracle_lib import Oracle
+    m, Y0, dt0, c = make_states("h2_small", 40)
+    ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
+    ev = Evaluator(m, ode, 0.0, max_cells=64)
+    rho = np.linspace(0.5, 1.5, 40)
+    RR, dtn, dtmin = ev.computeRR(c["deltaT"], 5e-7, rho, dt0, Y0.T.reshape(-1))
+    _, dto, dtmino, _, rro = Oracle(m).solve_cells(ode, Y0, c["deltaT"], dt0, rho=rho,
+                                                   want_rr=True, deltaTChemMax=5e-7)
+    RR = RR.reshape(40, m.n_species).T
+    scale = np.abs(rro).max()
+    assert np.allclose(RR, rro, rtol=1e-5, atol=1e-9 * scale)
+    assert np.allclose(dtn, dto, rtol=1e-6) and np.isclose(dtmin, dtmino, rtol=1e-6)
+    Ynew, dty = ev.computeYNew(c["deltaT"], 1e300, dt0, Y0.T.reshape(-1))
+    assert Ynew.shape == (40 * (m.n_species + 2),)
+
+
+def test_in_place_ld_and_empty():
+    from gpufoam_amd.evaluator import Evaluator
+    m, Y0, dt0, c = make_states("h2_small", 70)
+    ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
+    ev = Evaluator(m, ode, 0.0, max_cells=128)
+    a = ev.solve_host(c["deltaT"], Y0, dt0)
+    # leading dimension 100 > nCells, Ynew aliasing Y0
+    big = np.zeros((m.n_species + 2, 100))
+    big[:, :70] = Y0
+    dY = torch.tensor(big, device="cuda")
+    ddt = torch.tensor(dt0, device="cuda")
+    ev.solve(c["deltaT"], dY, ddt, Ynew=dY, ldY=100)
+    torch.cuda.synchronize()
+    assert np.array_equal(dY.cpu().numpy()[:, :70], a[0])
+    assert np.array_equal(ddt.cpu().numpy(), a[1])
+    # zero cells: deltaTMin = +inf, nothing else touched
+    dmin = torch.zeros(1, dtype=torch.float64, device="cuda")
+    ev.solve(c["deltaT"], dY[:, :0], ddt[:0], deltaTMin=dmin, ldY=100)
+    torch.cuda.synchronize()
+    assert dmin.item() == float("inf")
+    with pytest.raises(GfcError) as e:
+        ev.solve_host(c["deltaT"], np.zeros((m.n_species + 2, 200)), np.ones(200))
+    assert e.value.status == 9
+
+
+def test_max_steps_is_reported():
+    """Running out of maxSteps is silent in the reference
+    (gpuODESolverBase.H:80-111); here it sets a status bit."""
+    from gpufoam_amd.evaluator import Evaluator
+    from oracle_lib import Oracle
+    m, Y0, dt0, c = make_states("h2_small", 32)
+    ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"], maxSteps=3)
+    ref = Oracle(m).solve_cells(ode, Y0, c["deltaT"], dt0)
+    for batched in (False, True):
+        ev = Evaluator(m, ode, 0.0, max_cells=32)
+        Yn, dt, _, st, _ = ev.solve_host(c["deltaT"], Y0, dt0, batched=batched)
+        assert np.array_equal(st[0], ref[3][0])
+        assert (st[0] & 1).any()

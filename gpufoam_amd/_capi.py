@@ -101,6 +101,15 @@ def lib() -> C.CDLL:
                                     vp, vp, vp, vp, vp, vp, vp, vp, C.c_uint]),
             "gfc_evaluator_set_batched_shape": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
             "gfc_evaluator_get_batched_shape": (C.c_int, [vp, _ip, _ip, _ip]),
+            "gfc_evaluator_param_count": (C.c_int, [vp]),
+            "gfc_evaluator_get_rate_parameters": (C.c_int, [vp, _dp]),
+            "gfc_evaluator_set_rate_parameters": (C.c_int, [vp, _dp]),
+            "gfc_adjoint_set_checkpoint_interval": (C.c_int, [vp, C.c_int]),
+            "gfc_adjoint_checkpoint_interval": (C.c_int, [vp]),
+            "gfc_adjoint_forward": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, vp, vp, vp,
+                                              vp, vp]),
+            "gfc_adjoint_backward": (C.c_int, [vp, vp, C.c_int, vp, vp, vp]),
+            "gfc_adjoint_replay": (C.c_int, [vp, vp, C.c_int, vp, vp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name, None)
@@ -121,7 +130,10 @@ def exported_symbols() -> list[str]:
         "gfc_mechanism_species_index", "gfc_ode_inputs_from_foam",
         "gfc_solver_from_name", "gfc_ode_inputs_default", "gfc_evaluator_create",
         "gfc_evaluator_destroy", "gfc_solve", "gfc_evaluator_set_batched_shape",
-        "gfc_evaluator_get_batched_shape",
+        "gfc_evaluator_get_batched_shape", "gfc_evaluator_param_count",
+        "gfc_evaluator_get_rate_parameters", "gfc_evaluator_set_rate_parameters",
+        "gfc_adjoint_set_checkpoint_interval", "gfc_adjoint_checkpoint_interval",
+        "gfc_adjoint_forward", "gfc_adjoint_backward", "gfc_adjoint_replay",
     ]
 
 

@@ -1,0 +1,351 @@
+// Discrete adjoint of the per-cell chemistry solve (no counterpart in
+// gpuFoam).
+//
+//  1. adjoint_forward_kernel: the plain per-cell solve (one lane per cell,
+//     singleCellSolver.H:27-51) that also records, for every accepted step,
+//     its step size, and every K-th accepted step its initial state
+//     (checkpoints), plus the final state before the non-negativity clamp.
+//  2. adjoint_reverse_kernel: walks the accepted steps backwards.  For each
+//     segment of K steps it re-integrates the segment from its checkpoint
+//     with the recorded step sizes, then applies the reverse of each step
+//     (adjoint_math.hpp), which recomputes the step's Jacobian, LU and
+//     stages.  Gradients are those of the discrete map with the accepted
+//     step sizes held fixed.
+//  3. adjoint_replay_kernel: re-integrates the recorded step sequence from
+//     a (perturbed) initial state with the current rate parameters: the
+//     map the adjoint differentiates, for finite-difference checks.
+//
+// Memory: step sizes maxSteps x cells, checkpoints ceil(maxSteps/K) x n x
+// cells, per-cell scratch (forward + adjoint arrays + a K-state segment
+// buffer).  K is chosen automatically so that the checkpoints fit in
+// about 40 % of the free device memory, or set by the caller.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "adjoint_math.hpp"
+#include "evaluator.hpp"
+#include "mechanism.hpp"
+
+namespace gfc {
+
+namespace {
+
+bool hipFail(hipError_t e, const char* what) {
+    if (e == hipSuccess) return false;
+    setLastError(std::string(what) + ": " + hipGetErrorString(e));
+    return true;
+}
+
+struct AdjState {
+    long cap = 0;   // cells of workspace (multiple of 64)
+    int K = 0;      // checkpoint interval in use
+    int Kreq = 0;   // requested interval (0: automatic)
+    long laneD = 0, adjD = 0;
+    double* scratch = nullptr;
+    double* ck = nullptr;
+    double* hs = nullptr;
+    double* y0 = nullptr;
+    double* yEnd = nullptr;
+    int* nAcc = nullptr;
+    int nCells = 0;
+    double deltaT = 0;
+    bool valid = false;
+    void release() {
+        void* ptrs[] = {scratch, ck, hs, y0, yEnd, nAcc};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+        scratch = ck = hs = y0 = yEnd = nullptr;
+        nAcc = nullptr;
+        cap = 0;
+        valid = false;
+    }
+};
+
+AdjState* state(gfc_evaluator* ev) {
+    if (!ev->adjoint) ev->adjoint = new AdjState();
+    return static_cast<AdjState*>(ev->adjoint);
+}
+
+} // namespace
+
+// Forward solve with recording (one lane per cell).
+__global__ void __launch_bounds__(64)
+adjoint_forward_kernel(SolveArgs A, double* scratch, long stride, double* ck,
+                       double* hs, int K, int* nAcc, double* yEnd, double* y0copy) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= A.nCells) return;
+    const int nS = A.m.nS, n = nS + 1;
+    const lane::LaneBuf b{scratch + cell, stride, n};
+    const long ld = A.ldY;
+    for (int k = 0; k < nS + 2; ++k) y0copy[k * stride + cell] = A.Y0[k * ld + cell];
+    for (int k = 0; k < n; ++k) b.y()[k] = A.Y0[k * ld + cell];
+    const double p = A.Y0[(long)(nS + 1) * ld + cell];
+    double dtChem = A.dtChem[cell];
+    int attempts = 0, accepted = 0;
+    adj::Recorder rec{ck + cell, hs + cell, stride, n, K, 0};
+    const int status = adj::cell_forward(A, p, b, dtChem, rec, lane::SV{yEnd + cell, stride},
+                                         attempts, accepted);
+    nAcc[cell] = rec.count;
+    if (A.Ynew) {
+        for (int k = 0; k < n; ++k) A.Ynew[k * ld + cell] = b.y()[k];
+        A.Ynew[(long)(nS + 1) * ld + cell] = p;
+    }
+    A.dtChem[cell] = fmin(dtChem, A.deltaTChemMax);
+    if (A.stats) {
+        A.stats[cell] = status;
+        A.stats[ld + cell] = attempts;
+        A.stats[2 * ld + cell] = accepted;
+    }
+}
+
+// Reverse sweep (one lane per cell).  Ybar/Y0bar: [n][ldb]; th: [6 nR][ldb].
+__global__ void __launch_bounds__(64)
+adjoint_reverse_kernel(SolveArgs A, double* scratch, long stride, long laneD, long adjD,
+                       const double* ck, const double* hs, int K, const int* nAcc,
+                       double* yEnd, const double* y0copy, const double* Ybar,
+                       long ldb, double* Y0bar, double* thBar) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= A.nCells) return;
+    const int nS = A.m.nS, n = nS + 1;
+    lane::LaneBuf b{scratch + cell, stride, n};
+    adj::AdjBuf a{b, scratch + laneD * stride + cell, stride, n};
+    a.b.raw = a.mat(1);
+    a.b.jt = a.mat(2);
+    const double p = y0copy[(long)(nS + 1) * stride + cell];
+    const double T0 = y0copy[(long)nS * stride + cell];
+    lane::SV yb = a.ybar();
+    for (int e = 0; e < n; ++e) yb[e] = Ybar[e * ldb + cell];
+    const adj::ThView th{thBar ? thBar + cell : nullptr, ldb};
+    if (th.p)
+        for (long q = 0; q < (long)adj::kParams * A.m.nR; ++q) th.p[q * ldb] = 0.0;
+    adj::cell_reverse(A, p, T0, a, scratch + (laneD + adjD) * stride + cell, ck + cell,
+                      hs + cell, stride, K, nAcc[cell], lane::SV{yEnd + cell, stride}, th);
+    for (int e = 0; e < n; ++e) Y0bar[e * ldb + cell] = yb[e];
+}
+
+// Re-integrates the recorded accepted steps from Y0 (one lane per cell).
+__global__ void __launch_bounds__(64)
+adjoint_replay_kernel(SolveArgs A, double* scratch, long stride, const double* hs,
+                      const int* nAcc, const double* y0copy) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= A.nCells) return;
+    const int nS = A.m.nS, n = nS + 1;
+    const lane::LaneBuf b{scratch + cell, stride, n};
+    const long ld = A.ldY;
+    for (int k = 0; k < n; ++k) b.y()[k] = A.Y0[k * ld + cell];
+    const double p = A.Y0[(long)(nS + 1) * ld + cell];
+    const double T0 = y0copy[(long)nS * stride + cell];
+    if (T0 > A.Treact && A.deltaT > kSmall) {
+        const int N = nAcc[cell];
+        for (int t = 0; t < N; ++t) {
+            lane::derivatives(A.m, p, b.y(), b.dydx0(), b);
+            (void)lane::ros_step(A, p, b.y(), hs[(long)t * stride + cell], b.yTemp(), b);
+            for (int k = 0; k < n; ++k) b.y()[k] = b.yTemp()[k];
+        }
+        for (int i = 0; i < nS; i++) b.y()[i] = fmax(0.0, b.y()[i]);
+    }
+    for (int k = 0; k < n; ++k) A.Ynew[k * ld + cell] = b.y()[k];
+    A.Ynew[(long)(nS + 1) * ld + cell] = p;
+}
+
+void adjoint_free(gfc_evaluator* ev) {
+    auto* st = static_cast<AdjState*>(ev->adjoint);
+    if (!st) return;
+    st->release();
+    delete st;
+    ev->adjoint = nullptr;
+}
+
+// Sizes (and allocates) the workspace for nCells cells.
+static int adjoint_reserve(gfc_evaluator* ev, int nCells) {
+    AdjState* st = state(ev);
+    const long n = ev->nS + 1;
+    const long cap = ((long)nCells + 63) / 64 * 64;
+    const long maxSteps = std::max(1, ev->ode.maxSteps);
+    int K = st->Kreq;
+    if (K <= 0) {
+        size_t freeB = 0, totalB = 0;
+        if (hipFail(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo")) return GFC_ERR_HIP;
+        freeB += (size_t)st->cap * 8 *
+                 ((st->laneD + st->adjD + (long)st->K * n) + maxSteps + n + 2 * n + 2);
+        if (st->ck) freeB += (size_t)8 * ((maxSteps + st->K - 1) / std::max(1, st->K)) * n * st->cap;
+        const double budget = 0.4 * (double)freeB;
+        const double full = 8.0 * (double)maxSteps * n * cap;
+        K = full <= budget ? 1 : (int)std::ceil(full / budget);
+        K = std::min<long>(K, maxSteps);
+    }
+    if (st->cap >= cap && st->K == K && st->scratch) return GFC_OK;
+    st->release();
+    st->laneD = lane::lane_scratch_doubles(ev->nS);
+    st->adjD = adj::adj_scratch_doubles(ev->nS);
+    st->K = K;
+    st->cap = cap;
+    const long nck = (maxSteps + K - 1) / K;
+    const size_t scratchB = (size_t)8 * (st->laneD + st->adjD + (long)K * n) * cap;
+    if (hipFail(hipMalloc(&st->scratch, scratchB), "hipMalloc(adjoint scratch)") ||
+        hipFail(hipMalloc(&st->ck, (size_t)8 * nck * n * cap), "hipMalloc(checkpoints)") ||
+        hipFail(hipMalloc(&st->hs, (size_t)8 * maxSteps * cap), "hipMalloc(step sizes)") ||
+        hipFail(hipMalloc(&st->y0, (size_t)8 * (n + 1) * cap), "hipMalloc(adjoint y0)") ||
+        hipFail(hipMalloc(&st->yEnd, (size_t)8 * n * cap), "hipMalloc(adjoint yEnd)") ||
+        hipFail(hipMalloc(&st->nAcc, (size_t)4 * cap), "hipMalloc(adjoint counts)")) {
+        st->release();
+        return GFC_ERR_HIP;
+    }
+    return GFC_OK;
+}
+
+} // namespace gfc
+
+using namespace gfc;
+
+extern "C" {
+
+int gfc_evaluator_param_count(const gfc_evaluator* ev) {
+    return ev ? adj::kParams * ev->nR : 0;
+}
+
+int gfc_evaluator_get_rate_parameters(const gfc_evaluator* ev, double* params) {
+    if (!ev || !params) {
+        setLastError("gfc_evaluator_get_rate_parameters: bad argument");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    const gfc_mechanism& m = ev->mech;
+    for (int r = 0; r < m.nR; ++r) {
+        double* q = params + adj::kParams * r;
+        q[0] = std::log(std::fabs(m.k[3 * r]));
+        q[1] = m.k[3 * r + 1];
+        q[2] = m.k[3 * r + 2];
+        const bool fo = m.type[r] == GFC_RATE_LINDEMANN || m.type[r] == GFC_RATE_TROE;
+        q[3] = fo ? std::log(std::fabs(m.kInf[3 * r])) : 0.0;
+        q[4] = fo ? m.kInf[3 * r + 1] : 0.0;
+        q[5] = fo ? m.kInf[3 * r + 2] : 0.0;
+    }
+    return GFC_OK;
+}
+
+int gfc_evaluator_set_rate_parameters(gfc_evaluator* ev, const double* params) {
+    if (!ev || !params) {
+        setLastError("gfc_evaluator_set_rate_parameters: bad argument");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    gfc_mechanism& m = ev->mech;
+    for (int r = 0; r < m.nR; ++r) {
+        const double* q = params + adj::kParams * r;
+        const double s0 = m.k[3 * r] < 0 ? -1.0 : 1.0;
+        m.k[3 * r] = s0 * std::exp(q[0]);
+        m.k[3 * r + 1] = q[1];
+        m.k[3 * r + 2] = q[2];
+        if (m.type[r] == GFC_RATE_LINDEMANN || m.type[r] == GFC_RATE_TROE) {
+            const double s1 = m.kInf[3 * r] < 0 ? -1.0 : 1.0;
+            m.kInf[3 * r] = s1 * std::exp(q[3]);
+            m.kInf[3 * r + 1] = q[4];
+            m.kInf[3 * r + 2] = q[5];
+        }
+    }
+    if (hipFail(hipDeviceSynchronize(), "hipDeviceSynchronize")) return GFC_ERR_HIP;
+    return evaluator_upload(ev);
+}
+
+int gfc_adjoint_set_checkpoint_interval(gfc_evaluator* ev, int K) {
+    if (!ev || K < 0) {
+        setLastError("gfc_adjoint_set_checkpoint_interval: bad argument");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    state(ev)->Kreq = K;
+    return GFC_OK;
+}
+
+int gfc_adjoint_checkpoint_interval(const gfc_evaluator* ev) {
+    if (!ev || !ev->adjoint) return 0;
+    return static_cast<const AdjState*>(ev->adjoint)->K;
+}
+
+int gfc_adjoint_forward(gfc_evaluator* ev, double deltaT, int nCells, int ldY,
+                        const double* Y0, double* deltaTChem, double* Ynew,
+                        int* cellStats, void* stream) {
+    if (!ev || nCells < 0 || ldY < nCells || !(deltaT > 0) ||
+        (nCells > 0 && (!Y0 || !deltaTChem))) {
+        setLastError("gfc_adjoint_forward: bad argument");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    if (nCells > ev->maxCells) {
+        setLastError("gfc_adjoint_forward: nCells exceeds the evaluator's maxCells");
+        return GFC_ERR_CAPACITY;
+    }
+    AdjState* st = state(ev);
+    st->valid = false;
+    if (nCells == 0) {
+        st->nCells = 0;
+        st->deltaT = deltaT;
+        st->valid = true;
+        return GFC_OK;
+    }
+    int rc = adjoint_reserve(ev, nCells);
+    if (rc != GFC_OK) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    SolveArgs A = solve_args(ev, deltaT, 1e300, nCells);
+    A.ldY = ldY;
+    A.Y0 = Y0;
+    A.Ynew = Ynew;
+    A.dtChem = deltaTChem;
+    A.stats = cellStats;
+    const int blocks = (nCells + 63) / 64;
+    hipLaunchKernelGGL(adjoint_forward_kernel, dim3(blocks), dim3(64), 0, s, A, st->scratch,
+                       st->cap, st->ck, st->hs, st->K, st->nAcc, st->yEnd, st->y0);
+    if (hipFail(hipGetLastError(), "adjoint forward launch")) return GFC_ERR_HIP;
+    st->nCells = nCells;
+    st->deltaT = deltaT;
+    st->valid = true;
+    return GFC_OK;
+}
+
+int gfc_adjoint_backward(gfc_evaluator* ev, const double* Ybar, int ldb, double* Y0bar,
+                         double* paramBar, void* stream) {
+    if (!ev || !ev->adjoint || !static_cast<AdjState*>(ev->adjoint)->valid) {
+        setLastError("gfc_adjoint_backward: no recorded forward solve");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    AdjState* st = state(ev);
+    if (st->nCells == 0) return GFC_OK;
+    if (!Ybar || !Y0bar || ldb < st->nCells) {
+        setLastError("gfc_adjoint_backward: bad argument");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    SolveArgs A = solve_args(ev, st->deltaT, 1e300, st->nCells);
+    const int blocks = (st->nCells + 63) / 64;
+    hipLaunchKernelGGL(adjoint_reverse_kernel, dim3(blocks), dim3(64), 0, s, A, st->scratch,
+                       st->cap, st->laneD, st->adjD, st->ck, st->hs, st->K, st->nAcc,
+                       st->yEnd, st->y0, Ybar, (long)ldb, Y0bar, paramBar);
+    if (hipFail(hipGetLastError(), "adjoint reverse launch")) return GFC_ERR_HIP;
+    return GFC_OK;
+}
+
+int gfc_adjoint_replay(gfc_evaluator* ev, const double* Y0, int ldY, double* Ynew,
+                       void* stream) {
+    if (!ev || !ev->adjoint || !static_cast<AdjState*>(ev->adjoint)->valid) {
+        setLastError("gfc_adjoint_replay: no recorded forward solve");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    AdjState* st = state(ev);
+    if (st->nCells == 0) return GFC_OK;
+    if (!Y0 || !Ynew || ldY < st->nCells) {
+        setLastError("gfc_adjoint_replay: bad argument");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    SolveArgs A = solve_args(ev, st->deltaT, 1e300, st->nCells);
+    A.ldY = ldY;
+    A.Y0 = Y0;
+    A.Ynew = Ynew;
+    const int blocks = (st->nCells + 63) / 64;
+    hipLaunchKernelGGL(adjoint_replay_kernel, dim3(blocks), dim3(64), 0, s, A, st->scratch,
+                       st->cap, st->hs, st->nAcc, st->y0);
+    if (hipFail(hipGetLastError(), "adjoint replay launch")) return GFC_ERR_HIP;
+    return GFC_OK;
+}
+
+} // extern "C"

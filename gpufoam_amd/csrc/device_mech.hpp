@@ -92,32 +92,32 @@ enum RecI {
 
 // ---------------------------------------------------------------- thermo --
 // gpuThermo.H:29-32
-__device__ __forceinline__ const double* coeffs_of(const double* high,
+GFC_HD __forceinline__ const double* coeffs_of(const double* high,
                                                    const double* low,
                                                    double Tcommon, double T) {
     return (T < Tcommon) ? low : high;
 }
 // gpuThermo.H:177-184
-__device__ __forceinline__ double poly_Cp(const double* a, double T) {
+GFC_HD __forceinline__ double poly_Cp(const double* a, double T) {
     return ((((a[4] * T + a[3]) * T + a[2]) * T + a[1]) * T + a[0]);
 }
 // gpuThermo.H:210-221
-__device__ __forceinline__ double poly_Ha(const double* a, double T) {
+GFC_HD __forceinline__ double poly_Ha(const double* a, double T) {
     return (((((a[4] / 5.0 * T + a[3] / 4.0) * T + a[2] / 3.0) * T +
               a[1] / 2.0) * T + a[0]) * T + a[5]);
 }
 // gpuThermo.H:333-338
-__device__ __forceinline__ double poly_dCpdT(const double* a, double T) {
+GFC_HD __forceinline__ double poly_dCpdT(const double* a, double T) {
     return (((4.0 * a[4] * T + 3.0 * a[3]) * T + 2.0 * a[2]) * T + a[1]);
 }
 // gpuThermo.H:261-271 (the perfect-gas entropy term at pStd is zero)
-__device__ __forceinline__ double poly_S(const double* a, double T,
+GFC_HD __forceinline__ double poly_S(const double* a, double T,
                                          double logT) {
     return ((((a[4] / 4.0 * T + a[3] / 3.0) * T + a[2] / 2.0) * T + a[1]) *
                 T + a[0] * logT + a[6]);
 }
 // gpuThermo.H:287-299
-__device__ __forceinline__ double poly_Gstd(const double* a, double T,
+GFC_HD __forceinline__ double poly_Gstd(const double* a, double T,
                                             double logT) {
     return ((a[0] * (1.0 - logT) -
              (((a[4] / 20.0 * T + a[3] / 12.0) * T + a[2] / 6.0) * T +
@@ -126,7 +126,7 @@ __device__ __forceinline__ double poly_Gstd(const double* a, double T,
 
 // ----------------------------------------------------------------- rates --
 // gpuArrheniusReactionRate.H:16-26
-__device__ __forceinline__ double arrhenius(const double* k, double T) {
+GFC_HD __forceinline__ double arrhenius(const double* k, double T) {
     double ak = k[0];
     if (fabs(k[1]) > kVSmall) ak *= pow(T, k[1]);
     if (fabs(k[2]) > kVSmall) ak *= exp(-k[2] / T);
@@ -134,7 +134,7 @@ __device__ __forceinline__ double arrhenius(const double* k, double T) {
 }
 
 // gpuTroeFallOffFunction.H:27-47
-__device__ __forceinline__ double troe_F(const double* tr, double T,
+GFC_HD __forceinline__ double troe_F(const double* tr, double T,
                                          double Pr) {
     const double alpha = tr[0], Tsss = tr[1], Ts = tr[2], Tss = tr[3];
     const double logPr = log10(fmax(Pr, kSmall));
@@ -152,7 +152,7 @@ __device__ __forceinline__ double troe_F(const double* tr, double T,
 }
 
 // gpuSpecieExponent.H:21-36
-__device__ __forceinline__ double integer_pow(double x, int e) {
+GFC_HD __forceinline__ double integer_pow(double x, int e) {
     if (e == 0) return 1;
     double xx = e > 0 ? x : 1 / x;
     double y = 1;
@@ -163,17 +163,17 @@ __device__ __forceinline__ double integer_pow(double x, int e) {
     return y;
 }
 // gpuSpecieExponent.H:103-110
-__device__ __forceinline__ double specie_pow(double x, int ei, double er) {
+GFC_HD __forceinline__ double specie_pow(double x, int ei, double er) {
     if (ei != kNoIntExp) return integer_pow(x, ei);
     return pow(x, er);
 }
-__device__ __forceinline__ double specie_pow_m1(double x, int ei, double er) {
+GFC_HD __forceinline__ double specie_pow_m1(double x, int ei, double er) {
     if (ei != kNoIntExp) return integer_pow(x, ei - 1);
     return pow(x, er + (-1.0));
 }
 
 // Kc of reaction r at T (gpuThermo.H:80-113), as omega() evaluates it.
-__device__ __forceinline__ double reaction_Kc(const DeviceMech& m, int r,
+GFC_HD __forceinline__ double reaction_Kc(const DeviceMech& m, int r,
                                               double T) {
     const double Y = m.rY[r], W = m.rW[r];
     const double* a = coeffs_of(m.rhigh + 7 * r, m.rlow + 7 * r,
@@ -186,7 +186,7 @@ __device__ __forceinline__ double reaction_Kc(const DeviceMech& m, int r,
 }
 
 // KcAnddKcTbyKc (gpuThermo.H:139-169)
-__device__ __forceinline__ void reaction_Kc_dKc(const DeviceMech& m, int r,
+GFC_HD __forceinline__ void reaction_Kc_dKc(const DeviceMech& m, int r,
                                                 double T, double& Kc,
                                                 double& dKcdTbyKc) {
     const double Y = m.rY[r], W = m.rW[r];

@@ -221,8 +221,34 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     return GFC_OK;
 }
 
-int evaluator_upload(gfc_evaluator* ev, const gfc_mechanism* mech) {
-    const gfc_mech_view v = mech->view();
+SolveArgs solve_args(const gfc_evaluator* ev, double deltaT, double deltaTChemMax,
+                     int nCells) {
+    SolveArgs A{};
+    A.m = ev->dm;
+    A.tab = tableau(ev->ode.solver);
+    A.absTol = ev->ode.absTol;
+    A.relTol = ev->ode.relTol;
+    A.maxSteps = ev->ode.maxSteps;
+    A.safeScale = ev->ode.safeScale;
+    A.alphaInc = ev->ode.alphaInc;
+    A.alphaDec = ev->ode.alphaDec;
+    A.minScale = ev->ode.minScale;
+    A.maxScale = ev->ode.maxScale;
+    A.incThreshold = std::pow(ev->ode.maxScale / ev->ode.safeScale, -1.0 / ev->ode.alphaInc);
+    A.Treact = ev->Treact;
+    A.deltaT = deltaT;
+    A.deltaTChemMax = deltaTChemMax;
+    A.nCells = nCells;
+    return A;
+}
+
+// Uploads both table orders from the evaluator's copy of the mechanism
+// (replacing earlier uploads).
+int evaluator_upload(gfc_evaluator* ev) {
+    if (ev->mechBlock) (void)hipFree(ev->mechBlock);
+    if (ev->mechBlockTeam) (void)hipFree(ev->mechBlockTeam);
+    ev->mechBlock = ev->mechBlockTeam = nullptr;
+    const gfc_mech_view v = ev->mech.view();
     const int nR = v.n_reactions;
     // reference order for the per-lane kernel
     std::vector<int> ident(nR);
@@ -266,7 +292,8 @@ int gfc_evaluator_create(const gfc_mechanism* mech, const gfc_ode_inputs* ode,
     ev->ode = *ode;
     ev->Treact = Treact;
     hipGetDevice(&ev->device);
-    int st = evaluator_upload(ev, mech);
+    ev->mech = *mech;
+    int st = evaluator_upload(ev);
     if (st != GFC_OK) {
         gfc_evaluator_destroy(ev);
         return st;
@@ -300,6 +327,7 @@ int gfc_evaluator_create(const gfc_mechanism* mech, const gfc_ode_inputs* ode,
 void gfc_evaluator_destroy(gfc_evaluator* ev) {
     if (!ev) return;
     batched_free(ev);
+    adjoint_free(ev);
     void* ptrs[] = {ev->mechBlock, ev->mechBlockTeam, ev->laneScratch, ev->minBits, ev->sY0,
                     ev->sYnew, ev->sRR, ev->sRho, ev->sDt, ev->sDtMin,
                     ev->sStats};
@@ -338,22 +366,7 @@ int gfc_solve(gfc_evaluator* ev, double deltaT, double deltaTChemMax,
     const int nS = ev->nS, nE = nS + 2;
     const bool host = flags & GFC_HOST_POINTERS;
 
-    SolveArgs A{};
-    A.m = ev->dm;
-    A.tab = tableau(ev->ode.solver);
-    A.absTol = ev->ode.absTol;
-    A.relTol = ev->ode.relTol;
-    A.maxSteps = ev->ode.maxSteps;
-    A.safeScale = ev->ode.safeScale;
-    A.alphaInc = ev->ode.alphaInc;
-    A.alphaDec = ev->ode.alphaDec;
-    A.minScale = ev->ode.minScale;
-    A.maxScale = ev->ode.maxScale;
-    A.incThreshold = std::pow(ev->ode.maxScale / ev->ode.safeScale, -1.0 / ev->ode.alphaInc);
-    A.Treact = ev->Treact;
-    A.deltaT = deltaT;
-    A.deltaTChemMax = deltaTChemMax;
-    A.nCells = nCells;
+    SolveArgs A = solve_args(ev, deltaT, deltaTChemMax, nCells);
 
     if (host) {
         // stage host buffers through the evaluator's device buffers

@@ -5,6 +5,7 @@
 
 #include "../../include/gpufoam_amd.h"
 #include "kernel_common.hpp"
+#include "mechanism.hpp"
 
 struct gfc_evaluator {
     int device = 0;
@@ -35,9 +36,15 @@ struct gfc_evaluator {
     int wavesPerBlock = 0;
     int chunk = 0;
     void* batched = nullptr; // owned state of the batched launcher
+    void* adjoint = nullptr; // owned state of the discrete adjoint
+    gfc_mechanism mech;      // host copy of the mechanism (rate parameters can be updated)
 };
 
 namespace gfc {
+SolveArgs solve_args(const gfc_evaluator* ev, double deltaT, double deltaTChemMax,
+                     int nCells);
+int evaluator_upload(gfc_evaluator* ev);
+void adjoint_free(gfc_evaluator* ev);
 int batched_init(gfc_evaluator* ev);
 void batched_free(gfc_evaluator* ev);
 int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,

@@ -115,6 +115,60 @@ class Evaluator:
             _ptr(Y0), _ptr(Yn), _ptr(rr), _ptr(dt), _ptr(dtmin), _ptr(stats), None, flags))
         return Yn, dt, dtmin[0], stats, rr
 
+    # ---------------------------------------------------- rate parameters
+    @property
+    def n_params(self) -> int:
+        return _capi.lib().gfc_evaluator_param_count(self._h)
+
+    def get_rate_parameters(self) -> np.ndarray:
+        """[nR, 6]: ln|A|, beta, Ta of k (k0 for fall-off) and of kInf."""
+        out = np.zeros(self.n_params)
+        _capi.check(_capi.lib().gfc_evaluator_get_rate_parameters(
+            self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.reshape(-1, 6)
+
+    def set_rate_parameters(self, params) -> None:
+        a = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+        if a.size != self.n_params:
+            raise ValueError(f"expected {self.n_params} parameters, got {a.size}")
+        _capi.check(_capi.lib().gfc_evaluator_set_rate_parameters(
+            self._h, a.ctypes.data_as(C.POINTER(C.c_double))))
+
+    # ------------------------------------------------------------ adjoint
+    def set_checkpoint_interval(self, K: int) -> None:
+        _capi.check(_capi.lib().gfc_adjoint_set_checkpoint_interval(self._h, K))
+
+    @property
+    def checkpoint_interval(self) -> int:
+        return _capi.lib().gfc_adjoint_checkpoint_interval(self._h)
+
+    def adjoint_forward(self, deltaT: float, Y0, dtChem, Ynew=None, stats=None,
+                        stream=None) -> None:
+        """Recording forward solve on device tensors (SoA [nS+2, ld])."""
+        n = dtChem.shape[0]
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        _capi.check(_capi.lib().gfc_adjoint_forward(
+            self._h, deltaT, n, Y0.shape[1], _ptr(Y0), _ptr(dtChem), _ptr(Ynew),
+            _ptr(stats), stream))
+
+    def adjoint_backward(self, Ybar, Y0bar, paramBar=None, stream=None) -> None:
+        """Ybar, Y0bar [nS+1, ld]; paramBar [6 nR, ld] or None."""
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        _capi.check(_capi.lib().gfc_adjoint_backward(
+            self._h, _ptr(Ybar), Ybar.shape[1], _ptr(Y0bar), _ptr(paramBar), stream))
+
+    def adjoint_replay(self, Y0, Ynew, stream=None) -> None:
+        """Re-integrates the recorded accepted steps from Y0 into Ynew."""
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        _capi.check(_capi.lib().gfc_adjoint_replay(
+            self._h, _ptr(Y0), Y0.shape[1], _ptr(Ynew), stream))
+
     # ------------------------------------------------ reference interface
     def computeYNew(self, deltaT: float, deltaTChemMax: float, deltaTChem, Y):
         """GpuKernelEvaluator::computeYNew (gpuKernelEvaluator.cu:35-63):

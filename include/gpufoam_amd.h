@@ -208,6 +208,50 @@ int gfc_evaluator_set_batched_shape(gfc_evaluator* ev, int cellsPerWave,
 int gfc_evaluator_get_batched_shape(const gfc_evaluator* ev, int* cellsPerWave,
                                     int* wavesPerBlock, int* chunk);
 
+/* ---- rate parameters ----------------------------------------------------
+ * No counterpart in the reference (its rates are fixed at construction,
+ * makeGpuReactions.H:90-142).  Six parameters per reaction, in reaction
+ * order: ln|A|, beta, Ta of the rate expression (k0 for fall-off rates),
+ * then ln|A|, beta, Ta of kInf (fall-off rates; 0 and ignored otherwise),
+ * with k = A T^beta exp(-Ta/T) and Ta = Ea/R (makeGpuReactionRate.H:9-14).
+ * Setting keeps the sign of A and re-uploads the tables (synchronises the
+ * device). */
+int gfc_evaluator_param_count(const gfc_evaluator* ev);
+int gfc_evaluator_get_rate_parameters(const gfc_evaluator* ev, double* params);
+int gfc_evaluator_set_rate_parameters(gfc_evaluator* ev, const double* params);
+
+/* ---- discrete adjoint -----------------------------------------------------
+ * No counterpart in the reference.  Gradients of the solve's final state
+ * with respect to the initial state (Y, T) and the rate parameters, for the
+ * discrete map with the accepted step sizes of the forward solve held fixed.
+ *
+ * gfc_adjoint_forward: gfc_solve's per-cell solve (device pointers, no RR,
+ * no deltaTMin, deltaTChem not clamped) that records the accepted steps.
+ * The workspace (step sizes, checkpoints every K accepted steps, per-cell
+ * scratch) is allocated by this call when it grows.
+ *
+ * gfc_adjoint_backward: for the last recorded forward,
+ *   Ybar     [(nS+1)*ldb]  dL/d(final Y_0..Y_nS-1, T)
+ *   Y0bar    [(nS+1)*ldb]  out: dL/d(initial Y_0..Y_nS-1, T)
+ *   paramBar [6*nR*ldb]    out (may be NULL): dL/dparams per cell, layout
+ *                          of gfc_evaluator_get_rate_parameters
+ * dL/dEa = dL/dTa / R.
+ *
+ * gfc_adjoint_replay: re-integrates the recorded accepted steps from Y0
+ * (same cells, same layout as the forward) with the current parameters:
+ * the map whose derivative gfc_adjoint_backward returns.
+ *
+ * All three are asynchronous on `stream`. */
+int gfc_adjoint_set_checkpoint_interval(gfc_evaluator* ev, int K); /* 0: auto */
+int gfc_adjoint_checkpoint_interval(const gfc_evaluator* ev);
+int gfc_adjoint_forward(gfc_evaluator* ev, double deltaT, int nCells, int ldY,
+                        const double* Y0, double* deltaTChem, double* Ynew,
+                        int* cellStats, void* stream);
+int gfc_adjoint_backward(gfc_evaluator* ev, const double* Ybar, int ldb,
+                         double* Y0bar, double* paramBar, void* stream);
+int gfc_adjoint_replay(gfc_evaluator* ev, const double* Y0, int ldY,
+                       double* Ynew, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

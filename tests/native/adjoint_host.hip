@@ -1,0 +1,217 @@
+// Host build of the per-lane physics and adjoint math (the same source the
+// device kernels run), for CPU tests of the discrete adjoint against finite
+// differences of the oracle.  Test-only; not part of the library.
+#include <cmath>
+#include <vector>
+
+#include "../../gpufoam_amd/csrc/adjoint_math.hpp"
+
+using namespace gfc;
+
+namespace {
+
+struct HostMech {
+    std::vector<double> R;
+    DeviceMech d{};
+    explicit HostMech(const gfc_mech_view* v) {
+        const int nS = v->n_species;
+        R.resize(nS);
+        for (int i = 0; i < nS; ++i) R[i] = kRR / v->W[i];
+        d.nS = nS;
+        d.nR = v->n_reactions;
+        d.Tlow = v->Tlow;
+        d.Thigh = v->Thigh;
+        d.W = v->W;
+        d.R = R.data();
+        d.Tcommon = v->sTcommon;
+        d.high = v->high;
+        d.low = v->low;
+        d.type = v->type;
+        d.irreversible = v->irreversible;
+        d.k = v->k;
+        d.kInf = v->kInf;
+        d.troe = v->troe;
+        d.eff = v->eff;
+        d.nl = v->n_lhs;
+        d.nr = v->n_rhs;
+        d.li = v->lhs_index;
+        d.ls = v->lhs_stoich;
+        d.le = v->lhs_exp;
+        d.lei = v->lhs_exp_int;
+        d.ri = v->rhs_index;
+        d.rs = v->rhs_stoich;
+        d.re = v->rhs_exp;
+        d.rei = v->rhs_exp_int;
+        d.rY = v->rY;
+        d.rW = v->rW;
+        d.rTcommon = v->rTcommon;
+        d.rhigh = v->rhigh;
+        d.rlow = v->rlow;
+    }
+};
+
+struct Work {
+    int n;
+    std::vector<double> lane, adj;
+    adj::AdjBuf a;
+    explicit Work(int nS)
+        : n(nS + 1), lane(lane::lane_scratch_doubles(nS) + 8), adj(adj::adj_scratch_doubles(nS)) {
+        lane::LaneBuf b{lane.data(), 1, n};
+        a = adj::AdjBuf{b, adj.data(), 1, n};
+        a.b.raw = a.mat(1);
+        a.b.jt = a.mat(2);
+    }
+};
+
+SolveArgs args_for(const DeviceMech& d, const gfc_ode_inputs* ode) {
+    SolveArgs A{};
+    A.m = d;
+    A.tab = tableau(ode->solver);
+    A.absTol = ode->absTol;
+    A.relTol = ode->relTol;
+    A.maxSteps = ode->maxSteps;
+    A.safeScale = ode->safeScale;
+    A.alphaInc = ode->alphaInc;
+    A.alphaDec = ode->alphaDec;
+    A.minScale = ode->minScale;
+    A.maxScale = ode->maxScale;
+    A.incThreshold = std::pow(ode->maxScale / ode->safeScale, -1.0 / ode->alphaInc);
+    return A;
+}
+
+void copy_in(lane::SV dst, const double* src, int n) {
+    for (int i = 0; i < n; ++i) dst[i] = src[i];
+}
+
+} // namespace
+
+extern "C" {
+
+// ybar += d<fbar, f(y)>/dy, th += d/dparams (th: [6 nR]).
+void adjh_vjp_f(const gfc_mech_view* v, double p, const double* y, const double* fbar,
+                double* ybar, double* th) {
+    HostMech hm(v);
+    Work w(v->n_species);
+    const int n = w.n;
+    std::vector<double> yy(y, y + n), fb(fbar, fbar + n);
+    adj::vjp_f(hm.d, p, lane::SV{yy.data(), 1}, lane::SV{fb.data(), 1}, lane::SV{ybar, 1},
+               adj::ThView{th, 1}, w.a);
+}
+
+// ybar += d/dy, th += d/dparams of <Jbar, J(y)> + <Fbar, F(y)> (J n x n).
+void adjh_jact(const gfc_mech_view* v, double p, const double* y, const double* Jbar,
+               const double* Fbar, double* ybar, double* th) {
+    HostMech hm(v);
+    Work w(v->n_species);
+    const int n = w.n;
+    std::vector<double> yy(y, y + n);
+    lane::SV ys{yy.data(), 1};
+    lane::jacobian(hm.d, p, ys, w.a.b);
+    for (int i = 0; i < n * n; ++i) w.a.b.jt[i] = w.a.b.J()[i];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) w.a.Jb(i, j) = Jbar[i * n + j];
+    copy_in(w.a.Fb(), Fbar, n);
+    adj::jact(hm.d, p, ys, lane::SV{ybar, 1}, adj::ThView{th, 1}, w.a);
+}
+
+// The lane Jacobian J (n x n) and F at y.
+void adjh_jacobian(const gfc_mech_view* v, double p, const double* y, double* J, double* F) {
+    HostMech hm(v);
+    Work w(v->n_species);
+    const int n = w.n;
+    std::vector<double> yy(y, y + n);
+    lane::jacobian(hm.d, p, lane::SV{yy.data(), 1}, w.a.b);
+    for (int i = 0; i < n * n; ++i) J[i] = w.a.b.J()[i];
+    for (int i = 0; i < n; ++i) F[i] = w.a.b.dfdx()[i];
+}
+
+// One step of size h from y0 (y1 out) and the reverse of it: ybar in/out.
+void adjh_step(const gfc_mech_view* v, const gfc_ode_inputs* ode, double p, const double* y0,
+               double h, double* y1) {
+    HostMech hm(v);
+    Work w(v->n_species);
+    const int n = w.n;
+    const SolveArgs A = args_for(hm.d, ode);
+    std::vector<double> yy(y0, y0 + n);
+    lane::SV ys{yy.data(), 1};
+    lane::derivatives(hm.d, p, ys, w.a.b.dydx0(), w.a.b);
+    (void)lane::ros_step(A, p, ys, h, lane::SV{y1, 1}, w.a.b);
+}
+
+void adjh_step_reverse(const gfc_mech_view* v, const gfc_ode_inputs* ode, double p,
+                       const double* y0, double h, double* ybar, double* th) {
+    HostMech hm(v);
+    Work w(v->n_species);
+    const int n = w.n;
+    const SolveArgs A = args_for(hm.d, ode);
+    std::vector<double> yy(y0, y0 + n);
+    copy_in(w.a.ybar(), ybar, n);
+    adj::step_reverse(A, p, lane::SV{yy.data(), 1}, h, adj::ThView{th, 1}, w.a);
+    for (int i = 0; i < n; ++i) ybar[i] = w.a.ybar()[i];
+}
+
+// A^T x = b with A's LU from lane::lu_decompose (A n x n row-major).
+void adjh_solve_T(int n, const double* Amat, double* x) {
+    std::vector<double> buf(14 * n + n * n + n + 8);
+    lane::LaneBuf b{buf.data(), 1, n};
+    for (int i = 0; i < n * n; ++i) b.J()[i] = Amat[i];
+    lane::lu_decompose(b);
+    adj::lu_solve_T(b, lane::SV{x, 1});
+}
+
+} // extern "C"
+
+extern "C" {
+
+// Whole-cell recording forward + reverse with checkpoint interval K (the
+// per-cell code of the adjoint kernels).  y: [nS+2] initial state (in) and
+// final state (out); ybar: [nS+1] dL/dfinal (in) and dL/dinitial (out);
+// th: [6 nR] out; hs: [maxSteps] recorded step sizes out.  Returns the
+// number of accepted steps (negative status bits on failure).
+int adjh_cell(const gfc_mech_view* v, const gfc_ode_inputs* ode, double Treact,
+              double deltaT, double* y, double* dtChem, int K, double* ybar, double* th,
+              double* hs) {
+    HostMech hm(v);
+    const int nS = v->n_species, n = nS + 1;
+    Work w(nS);
+    SolveArgs A = args_for(hm.d, ode);
+    A.Treact = Treact;
+    A.deltaT = deltaT;
+    const long maxSteps = ode->maxSteps;
+    std::vector<double> ck(((maxSteps + K - 1) / K + 1) * n), yEnd(n), seg((long)K * n);
+    copy_in(w.a.b.y(), y, n);
+    const double p = y[nS + 1], T0 = y[nS];
+    adj::Recorder rec{ck.data(), hs, 1, n, K, 0};
+    int attempts = 0, accepted = 0;
+    const int st = adj::cell_forward(A, p, w.a.b, *dtChem, rec, lane::SV{yEnd.data(), 1},
+                                     attempts, accepted);
+    for (int k = 0; k < n; ++k) y[k] = w.a.b.y()[k];
+    copy_in(w.a.ybar(), ybar, n);
+    for (long q = 0; q < 6L * v->n_reactions; ++q) th[q] = 0.0;
+    adj::cell_reverse(A, p, T0, w.a, seg.data(), ck.data(), hs, 1, K, rec.count,
+                      lane::SV{yEnd.data(), 1}, adj::ThView{th, 1});
+    for (int k = 0; k < n; ++k) ybar[k] = w.a.ybar()[k];
+    return st & ~GFC_CELL_SKIPPED ? -st : rec.count;
+}
+
+// Re-integrates N recorded steps from y (in/out, [nS+2]) and clamps.
+void adjh_replay(const gfc_mech_view* v, const gfc_ode_inputs* ode, double* y, int N,
+                 const double* hs) {
+    HostMech hm(v);
+    const int nS = v->n_species, n = nS + 1;
+    Work w(nS);
+    const SolveArgs A = args_for(hm.d, ode);
+    const double p = y[nS + 1];
+    lane::LaneBuf b = w.a.b;
+    b.raw = b.jt = nullptr;
+    copy_in(b.y(), y, n);
+    for (int t = 0; t < N; ++t) {
+        lane::derivatives(hm.d, p, b.y(), b.dydx0(), b);
+        (void)lane::ros_step(A, p, b.y(), hs[t], b.yTemp(), b);
+        for (int k = 0; k < n; ++k) b.y()[k] = b.yTemp()[k];
+    }
+    for (int i = 0; i < nS; i++) b.y()[i] = std::fmax(0.0, b.y()[i]);
+    for (int k = 0; k < n; ++k) y[k] = b.y()[k];
+}
+
+} // extern "C"

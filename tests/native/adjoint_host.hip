@@ -167,7 +167,8 @@ extern "C" {
 // per-cell code of the adjoint kernels).  y: [nS+2] initial state (in) and
 // final state (out); ybar: [nS+1] dL/dfinal (in) and dL/dinitial (out);
 // th: [6 nR] out; hs: [maxSteps] recorded step sizes out.  Returns the
-// number of accepted steps (negative status bits on failure).
+// number of accepted steps (a cell that runs out of maxSteps is
+// differentiated over the steps it took).
 int adjh_cell(const gfc_mech_view* v, const gfc_ode_inputs* ode, double Treact,
               double deltaT, double* y, double* dtChem, int K, double* ybar, double* th,
               double* hs) {
@@ -191,7 +192,8 @@ int adjh_cell(const gfc_mech_view* v, const gfc_ode_inputs* ode, double Treact,
     adj::cell_reverse(A, p, T0, w.a, seg.data(), ck.data(), hs, 1, K, rec.count,
                       lane::SV{yEnd.data(), 1}, adj::ThView{th, 1});
     for (int k = 0; k < n; ++k) ybar[k] = w.a.ybar()[k];
-    return st & ~GFC_CELL_SKIPPED ? -st : rec.count;
+    (void)st;
+    return rec.count;
 }
 
 // Re-integrates N recorded steps from y (in/out, [nS+2]) and clamps.

@@ -119,7 +119,11 @@ class Mechanism:
     @classmethod
     def load(cls, name: str) -> "Mechanism":
         """A bundled mechanism: 'h2' (10 species, 27 reactions), 'gri'
-        (GRI-Mech 3.0, 53 species, 325 reactions) or 'syn200'."""
+        (GRI-Mech 3.0, 53 species, 325 reactions) or 'syn200' (the seeded
+        synthetic 200-species / 1200-reaction mechanism, generated)."""
+        if name == "syn200":
+            from .synthetic import syn_mechanism
+            return syn_mechanism()
         return cls.from_dir(os.path.join(MECH_DIR, name), name)
 
     # ------------------------------------------------------------- writing

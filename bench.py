@@ -10,11 +10,12 @@ with the chemical time step reset to its initial value before each step so
 every step does the same work.  Inputs are resident in HBM.
 
 Printed (one JSON line): the headline metric and, beside it, the plain
-per-cell kernel (one lane per cell, the reference's mapping) on a sample of
-the same states, the FP64 roofline fraction of the batched kernel, and the
-CPU baseline: gpuFoam's own per-cell code built for the host (or the oracle
-restatement where that build is absent) on the host cores, on a bounded
-sample of the same states.
+per-cell kernel (one lane per cell, the reference's mapping) on the same
+cells, the FP64 roofline fraction of the batched kernel, the CPU baseline
+(gpuFoam's own per-cell code built for the host, or the oracle restatement
+where that build is absent, on the host cores over a bounded sample of the
+same states) and the adjoint/forward cost on the differentiation workload
+(10-species H2, 65536 cells, deltaT 1e-4).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--config gri_production]
 """
@@ -56,6 +57,52 @@ def cpu_baseline(config: str, sample: int, threads: int) -> dict:
                       f"({el:.1f} s wall on {threads} threads)"}
 
 
+def adjoint_cost(n_cells: int) -> dict:
+    """Adjoint/forward cost on the differentiation workload (H2, deltaT
+    1e-4): recording forward + reverse sweep versus the plain and the
+    batched forward solve of the same cells; loss sum(T + 1e3 Y_OH)."""
+    import torch
+    from gpufoam_amd import ode_inputs
+    from gpufoam_amd.evaluator import Evaluator
+    from gpufoam_amd.workloads import make_states
+    m, Y0, dt0, c = make_states("h2_adjoint", n_cells)
+    ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
+    ev = Evaluator(m, ode, 0.0, max_cells=n_cells)
+    dY0 = torch.tensor(Y0, device="cuda:0")
+    dYn = torch.empty_like(dY0)
+    ddt0 = torch.tensor(dt0, device="cuda:0")
+    n = m.n_species + 1
+    W = torch.zeros((n, n_cells), dtype=torch.float64, device="cuda:0")
+    W[m.n_species] = 1.0
+    W[m.index("OH")] = 1e3
+    g = torch.empty_like(W)
+    pb = torch.empty((6 * m.n_reactions, n_cells), dtype=torch.float64, device="cuda:0")
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3
+
+    ddt = ddt0.clone()
+    t_plain = timed(lambda: ev.solve(c["deltaT"], dY0, ddt, Ynew=dYn))
+    ddt.copy_(ddt0)
+    t_batched = timed(lambda: ev.solve(c["deltaT"], dY0, ddt, Ynew=dYn, batched=True))
+    ddt.copy_(ddt0)
+    t_fwd = timed(lambda: ev.adjoint_forward(c["deltaT"], dY0, ddt, Ynew=dYn))
+    t_bwd = timed(lambda: ev.adjoint_backward(W, g, pb))
+    return {"workload": "h2_adjoint", "cells": n_cells, "deltaT": c["deltaT"],
+            "loss": "sum(T + 1e3*Y_OH)", "params": 6 * m.n_reactions,
+            "forward_plain_s": t_plain, "forward_batched_s": t_batched,
+            "adjoint_forward_s": t_fwd, "adjoint_reverse_s": t_bwd,
+            "checkpoint_interval": ev.checkpoint_interval,
+            "adjoint_over_plain_forward": (t_fwd + t_bwd) / t_plain,
+            "adjoint_over_batched_forward": (t_fwd + t_bwd) / t_batched}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +116,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--adjoint-cells", type=int, default=65536,
+                    help="cells of the differentiation workload for the adjoint cost (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times "
                          "(the known-size kernel that scales the HBM byte counters)")
@@ -157,6 +206,8 @@ def main():
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cpu = None if args.no_cpu else cpu_baseline(args.config, min(args.cpu_sample, n), threads)
 
+    adjoint = adjoint_cost(args.adjoint_cells) if args.adjoint_cells else None
+
     traffic = None
     tfile = os.path.join(HERE, "profiles", f"hbm_traffic_{args.config}.json")
     if os.path.exists(tfile):
@@ -188,6 +239,7 @@ def main():
         "plain_cells_per_s": plain_value,
         "plain_sample_cells": ns,
         "batched_over_plain": value / plain_value if plain_value else None,
+        "adjoint": adjoint,
         "attempts_per_cell": float(st[1].mean()),
         "accepted_per_cell": float(st[2].mean()),
         "wall_s": wall,

@@ -1,0 +1,53 @@
+"""Launch-shape autotuner of the batched solve.
+
+The search space is the evaluator's batched shapes (gfc_evaluator_set_batched_shape):
+one wavefront per cell with 1, 2, 4, ... resident wavefronts per compute
+unit up to what fits, and one lane per cell.  Each shape is timed with HIP
+events on one CFD step of a sample of the caller's own cells (the chemical
+time step reset before every run, so every shape does the same work), and
+the fastest is set on the evaluator.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def autotune(ev, deltaT: float, Y0, dtChem, sample: int = 4096, repeats: int = 2,
+             seed: int = 0, shapes=None) -> dict:
+    """Times the batched solve for each launch shape on up to `sample`
+    cells drawn from Y0 [nS+2, ld] / dtChem [n] (torch device tensors) and
+    sets the fastest on `ev`.  Returns {(cellsPerWave, wavesPerCU): ms}
+    and the chosen shape under the key "best"."""
+    import torch
+    n = dtChem.shape[0]
+    if sample and sample < n:
+        idx = torch.as_tensor(np.sort(np.random.default_rng(seed).choice(n, sample,
+                                                                          replace=False)),
+                              device=Y0.device)
+        Ys = Y0[:, :n].index_select(1, idx).contiguous()
+        dts = dtChem.index_select(0, idx).contiguous()
+    else:
+        Ys, dts = Y0[:, :n].contiguous(), dtChem.clone()
+    Yn = torch.empty_like(Ys)
+    if shapes is None:
+        shapes = [(1, w) for w in (1, 2, 4, 8, 16)] + [(1, 0), (64, 0)]
+    times = {}
+    for cpw, wpc in shapes:
+        ev.set_batched_shape(cpw, wpc)
+        d = dts.clone()
+        ev.solve(deltaT, Ys, d, Ynew=Yn, batched=True)  # warm-up (and graph-free)
+        ms = []
+        for _ in range(repeats):
+            d.copy_(dts)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            ev.solve(deltaT, Ys, d, Ynew=Yn, batched=True)
+            e1.record()
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        times[(cpw, wpc)] = min(ms)
+    best = min(times, key=times.get)
+    ev.set_batched_shape(*best)
+    times["best"] = best
+    return times

@@ -9,9 +9,9 @@
 //    replayed on the caller's stream on later calls with identical
 //    arguments; changed arguments update the graph in place.  Called inside
 //    the caller's own stream capture, the launches join that graph instead.
-//  * Launch shape (autotuned by gpufoam_amd.autotune): cells per wavefront
-//    1 (the wavefront kernel) or 64 (one lane per cell), and the number of
-//    wavefronts kept resident per compute unit.
+//  * Launch shape (autotuned by gpufoam_amd/autotune.py): cells per
+//    wavefront 1 (the wavefront kernel) or 64 (one lane per cell), and the
+//    number of wavefronts kept resident per compute unit.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -189,9 +189,9 @@ extern "C" {
 int gfc_evaluator_set_batched_shape(gfc_evaluator* ev, int cellsPerWave,
                                     int wavesPerBlock, int chunk) {
     if (!ev || (cellsPerWave != 0 && cellsPerWave != 1 && cellsPerWave != 64) ||
-        wavesPerBlock < 0 || chunk < 0) {
+        wavesPerBlock < 0 || chunk < 0 || chunk > 1) {
         gfc::setLastError("gfc_evaluator_set_batched_shape: unsupported shape "
-                          "(cellsPerWave must be 0, 1 or 64)");
+                          "(cellsPerWave must be 0, 1 or 64, chunk 0 or 1)");
         return GFC_ERR_INVALID_ARGUMENT;
     }
     ev->cellsPerWave = cellsPerWave;

@@ -201,8 +201,12 @@ int gfc_solve(gfc_evaluator* ev, double deltaT, double deltaTChemMax,
               int* cellStats, void* stream, unsigned flags);
 
 /* Launch-shape control for the persistent batched form (the autotuner's
- * search space).  cellsPerWave in {1,4,16,64} where the mechanism size
- * allows; wavesPerBlock in {1,2,4}; chunk = cells taken per queue grab. */
+ * search space, gpufoam_amd/autotune.py).  cellsPerWave: 0 = automatic
+ * (one wavefront per cell where the mechanism fits, else one lane per
+ * cell), 1 = one wavefront per cell, 64 = one lane per cell.
+ * wavesPerBlock: resident wavefronts per compute unit of the persistent
+ * grid (0 = as many as fit).  chunk: cells taken per queue grab (0 or 1;
+ * larger values are reserved). */
 int gfc_evaluator_set_batched_shape(gfc_evaluator* ev, int cellsPerWave,
                                     int wavesPerBlock, int chunk);
 int gfc_evaluator_get_batched_shape(const gfc_evaluator* ev, int* cellsPerWave,

@@ -35,6 +35,11 @@ sys.path.insert(0, HERE)
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 
 
+def progress(msg: str) -> None:
+    """Stage line on stderr (stdout carries only the JSON result)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(config: str, sample: int, threads: int) -> dict:
     """gpuFoam's per-cell solver built for the host (oracle/_ref) or the
     oracle, timed on `sample` cells of the same state law."""
@@ -89,11 +94,15 @@ def adjoint_cost(n_cells: int) -> dict:
 
     ddt = ddt0.clone()
     t_plain = timed(lambda: ev.solve(c["deltaT"], dY0, ddt, Ynew=dYn))
+    progress(f"adjoint workload: plain forward {t_plain:.2f} s")
     ddt.copy_(ddt0)
     t_batched = timed(lambda: ev.solve(c["deltaT"], dY0, ddt, Ynew=dYn, batched=True))
+    progress(f"adjoint workload: batched forward {t_batched:.2f} s")
     ddt.copy_(ddt0)
     t_fwd = timed(lambda: ev.adjoint_forward(c["deltaT"], dY0, ddt, Ynew=dYn))
+    progress(f"adjoint workload: recording forward {t_fwd:.2f} s")
     t_bwd = timed(lambda: ev.adjoint_backward(W, g, pb))
+    progress(f"adjoint workload: reverse sweep {t_bwd:.2f} s")
     return {"workload": "h2_adjoint", "cells": n_cells, "deltaT": c["deltaT"],
             "loss": "sum(T + 1e3*Y_OH)", "params": 6 * m.n_reactions,
             "forward_plain_s": t_plain, "forward_batched_s": t_batched,
@@ -164,6 +173,7 @@ def main():
                  stats=stats if with_stats else None, batched=True, graph=graph)
 
     # first call with statistics (work accounting), then warmup
+    progress(f"{args.config}: {n} cells, statistics run")
     step(with_stats=True)
     torch.cuda.synchronize()
     st = stats.cpu().numpy()
@@ -181,6 +191,7 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = e0.elapsed_time(e1) / args.steps
     value = n / (ms_per_step / 1e3)
+    progress(f"batched: {ms_per_step:.1f} ms per step")
 
     stages = {12: 2, 23: 3, 34: 4}[ode.solver]
     flops = work_flops(m, st[1], st[2], stages)
@@ -202,11 +213,17 @@ def main():
         p1.record(stream)
         torch.cuda.synchronize()
         plain_value = ns / (p0.elapsed_time(p1) / 1e3)
+        progress(f"plain: {plain_value:.0f} cells/s")
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cpu = None if args.no_cpu else cpu_baseline(args.config, min(args.cpu_sample, n), threads)
+    if cpu:
+        progress(f"cpu baseline: {cpu['value']:.0f} cells/s on {threads} threads")
 
-    adjoint = adjoint_cost(args.adjoint_cells) if args.adjoint_cells else None
+    adjoint = None
+    if args.adjoint_cells:
+        progress(f"adjoint cost on {args.adjoint_cells} cells of h2_adjoint")
+        adjoint = adjoint_cost(args.adjoint_cells)
 
     traffic = None
     tfile = os.path.join(HERE, "profiles", f"hbm_traffic_{args.config}.json")

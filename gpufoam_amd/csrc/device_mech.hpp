@@ -59,6 +59,9 @@ struct DeviceMech {
     // distinct third-body efficiency vectors: effSets[s*64 + i], and the
     // set of each reaction (-1 for plain Arrhenius)
     const double* effSets;
+    // the first kSetsT sets transposed: effSetsT[i*kSetsT + s] (specie-major,
+    // so one specie's efficiencies of all sets are one scalar load)
+    const double* effSetsT;
     const int* effSet;
     int nSets;
     // wavefront kernel: one record per reaction, field-major within a pass
@@ -68,6 +71,8 @@ struct DeviceMech {
     const int* recI;
     int teamWidth; // widest reaction side (the record holds kTeamWidth)
 };
+
+constexpr int kSetsT = 16;
 
 // Record layout of the wavefront kernel (at most kTeamWidth species per side).
 constexpr int kTeamWidth = 3;

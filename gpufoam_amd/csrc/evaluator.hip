@@ -107,6 +107,9 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     std::vector<double> effSets(std::max<size_t>(1, sets.size()) * 64, 0.0);
     for (size_t q = 0; q < sets.size(); ++q)
         for (int i = 0; i < nS && i < 64; ++i) effSets[q * 64 + i] = sets[q][i];
+    std::vector<double> effSetsT((size_t)64 * kSetsT, 0.0);
+    for (size_t q = 0; q < sets.size() && q < (size_t)kSetsT; ++q)
+        for (int i = 0; i < nS && i < 64; ++i) effSetsT[(size_t)i * kSetsT + q] = sets[q][i];
     // wavefront-kernel records (only usable when no side is wider than
     // kTeamWidth)
     int teamWidth = 0;
@@ -185,6 +188,7 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     add(lnA.data(), lnA.size());
     add(sgnA.data(), sgnA.size());
     add(effSets.data(), effSets.size());
+    add(effSetsT.data(), effSetsT.size());
     add(effSet.data(), effSet.size());
     add(recD.data(), recD.size());
     add(recI.data(), recI.size());
@@ -212,6 +216,7 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     at(d.lnA);
     at(d.sgnA);
     at(d.effSets);
+    at(d.effSetsT);
     at(d.effSet);
     at(d.recD);
     at(d.recI);

@@ -11,8 +11,8 @@
 //     grouped by rate type so a pass of 64 runs one code path).  Rate
 //     constants are evaluated as exp(ln A + beta ln T - Ta/T) and the
 //     reverse rate as exp(... - ln Kc) with ln T, 1/T and ln(pStd/RT)
-//     computed once per cell; third-body concentrations are DPP reductions
-//     over the mechanism's distinct efficiency vectors.  Contributions to
+//     computed once per cell; third-body concentrations are computed once
+//     per distinct efficiency vector of the mechanism.  Contributions to
 //     dN/dt and to the Jacobian are accumulated in LDS with FP64 LDS
 //     atomics;
 //   * row-parallel LU: lane i holds row i of -J + I/(gamma*dx) in N
@@ -209,14 +209,19 @@ __device__ __forceinline__ double side_dpow(const int* idx, const int* ei,
     return v;
 }
 
-// Third-body concentrations of every efficiency set: M[s] = sum_i eff_si c_i.
+// Third-body concentrations of every efficiency set, M[s] = sum_i eff_si c_i
+// (gpuThirdBodyEfficiencies.H:21-27, summed in specie order): lane s sums
+// set s, so the sets are computed side by side.  L.c must hold c.
 template <int N>
-__device__ __forceinline__ void third_body(const DeviceMech& m, TeamLDS<N>& L,
-                                           double cl, int lane) {
-#pragma unroll 1
-    for (int s = 0; s < m.nSets; ++s) {
-        const double Ms = wave_sum(m.effSets[s * kLanes + lane] * cl);
-        if (lane == 0) L.M[s] = Ms;
+__device__ __forceinline__ void third_body(const DeviceMech& m, TeamLDS<N>& L, int lane) {
+    __syncthreads();
+    const int nS = m.nS;
+    for (int s = lane; s < m.nSets; s += kLanes) {
+        const double* e = m.effSets + s * kLanes;
+        double acc = 0.0;
+#pragma unroll 4
+        for (int i = 0; i < nS; ++i) acc += e[i] * L.c[i];
+        L.M[s] = acc;
     }
 }
 
@@ -319,7 +324,7 @@ __device__ __forceinline__ double team_derivatives(const DeviceMech& m, double p
     const double cl = cu * rhoM;
     L.c[lane] = cl;
     L.dN[lane] = 0.0;
-    third_body<N>(m, L, cl, lane);
+    third_body<N>(m, L, lane);
     // omega() evaluates the rates at T clipped to [Tlow, Thigh]
     const TFun f = tfun(fmin(fmax(T, m.Tlow), m.Thigh));
     specie_gibbs<N, false>(m, L, f, lane);
@@ -355,7 +360,7 @@ __device__ __forceinline__ double team_derivatives(const DeviceMech& m, double p
 // pressure row/column dropped) in L.J and returns this lane's component of
 // the derivative computed alongside (the reference's "dfdx").
 template <int N>
-__device__ __noinline__ double team_jacobian(const DeviceMech& m, double p, double T,
+__device__ __forceinline__ double team_jacobian(const DeviceMech& m, double p, double T,
                                              double yl, TeamLDS<N>& L, int lane) {
     constexpr int P = TeamLDS<N>::P;
     const int nS = m.nS, n = nS + 1;
@@ -385,11 +390,12 @@ __device__ __noinline__ double team_jacobian(const DeviceMech& m, double p, doub
         L.colA[lane] = rhoM / m.W[lane];
         L.colB[lane] = rhoM * (1.0 / (p / (m.R[lane] * T)));
     }
-    third_body<N>(m, L, cl, lane);
+    third_body<N>(m, L, lane);
     const TFun f = tfun(T);
     specie_gibbs<N, true>(m, L, f, lane);
     for (int e = lane; e < n * P; e += kLanes) L.J[e] = 0.0;
     const int nChunks = m.nSets > 0 ? (m.nSets + kMaxSets - 1) / kMaxSets : 1;
+    const bool fusedDense = m.nSets <= kSetsT;
     bool poisoned = false; // a fall-off rate with Cf - Cr/Kc not finite
 
 #pragma unroll 1
@@ -476,8 +482,11 @@ __device__ __noinline__ double team_jacobian(const DeviceMech& m, double p, doub
             }
         }
         __syncthreads();
-        // dense third-body part, column-parallel: J(i, j) += sum_s Wd[s][i] eff_s[j]
-        if (lane < nS) {
+        // dense third-body part, J(i, j) += sum_s Wd[s][i] eff_s[j]: with at
+        // most kSetsT sets it is added row-parallel inside the ddY_dcdY
+        // transform below; otherwise column-parallel here, one chunk of
+        // kMaxSets sets at a time
+        if (!fusedDense && lane < nS) {
             const int ns = min(kMaxSets, m.nSets - s0);
             double e[kMaxSets];
 #pragma unroll
@@ -487,8 +496,7 @@ __device__ __noinline__ double team_jacobian(const DeviceMech& m, double p, doub
             for (int i = 0; i < nS; ++i) {
                 double acc = 0.0;
 #pragma unroll
-                for (int s = 0; s < kMaxSets; ++s)
-                    if (s < ns) acc += L.Wd[s * kLanes + i] * e[s];
+                for (int s = 0; s < kMaxSets; ++s) acc += L.Wd[s * kLanes + i] * e[s];
                 L.J[i * P + lane] += acc;
             }
         }
@@ -528,11 +536,29 @@ __device__ __noinline__ double team_jacobian(const DeviceMech& m, double p, doub
         double* row = &L.J[lane * P];
         const double WiByrhoM = m.W[lane] / rhoM;
         double ddNdT = row[nS];
+        if (fusedDense) {
+            // this row's weight of every set; the sets' efficiencies of
+            // column j are uniform (scalar loads of the transposed table)
+            double w[kSetsT];
+#pragma unroll
+            for (int q = 0; q < kSetsT; ++q) w[q] = L.Wd[q * kLanes + lane];
+#pragma unroll 2
+            for (int j = 0; j < nS; ++j) {
+                const double* e = m.effSetsT + j * kSetsT;
+                double acc = 0.0;
+#pragma unroll
+                for (int q = 0; q < kSetsT; ++q) acc += w[q] * e[q];
+                const double tj = row[j] + acc;
+                ddNdT -= tj * L.c[j] * alphavM;
+                row[j] = WiByrhoM * tj * L.colA[j] + L.colB[j] * d;
+            }
+        } else {
 #pragma unroll 4
-        for (int j = 0; j < nS; ++j) {
-            const double tj = row[j];
-            ddNdT -= tj * L.c[j] * alphavM;
-            row[j] = WiByrhoM * tj * L.colA[j] + L.colB[j] * d;
+            for (int j = 0; j < nS; ++j) {
+                const double tj = row[j];
+                ddNdT -= tj * L.c[j] * alphavM;
+                row[j] = WiByrhoM * tj * L.colA[j] + L.colB[j] * d;
+            }
         }
         row[nS] = WiByrhoM * ddNdT + alphavM * d;
     }
@@ -675,8 +701,14 @@ team_solve_kernel(SolveArgs A, int* queue) {
                     // adaptiveSolve (gpuAdaptiveSolver.H:83-134)
                     double dx = stepDx;
                     const double T = readlane(y, nS);
-                    const double dydx0 = team_derivatives<N>(m, p, T, y, L, lane);
+                    // the Jacobian call evaluates the right-hand side at y0
+                    // alongside (its dfdx); it equals derivatives(y0) unless
+                    // T lies outside the reaction temperature limits, where
+                    // omega() clips T and the Jacobian does not
                     const double dfdx = team_jacobian<N>(m, p, T, y, L, lane);
+                    const double dydx0 = (T >= m.Tlow && T <= m.Thigh)
+                                             ? dfdx
+                                             : team_derivatives<N>(m, p, T, y, L, lane);
                     double err, yNew;
                     for (;;) {
                         // -J + I/(gamma dx), factorised in registers

@@ -603,9 +603,18 @@ __device__ __forceinline__ void team_lu(double (&row)[N], int n, int lane, int& 
         double key = (done || (lane >= n && j < n)) ? -1.0 : vv * fabs(row[j]);
         if (!(key >= 0.0)) key = -1.0;
         const double kmax = wave_max(key);
-        const int P = wave_max_i((key == kmax && !done) ? pos : -1);
-        const int pv = __builtin_amdgcn_readfirstlane(
-            (int)__builtin_ctzll(__ballot(pos == P && !done)));
+        // one candidate (the rule case) needs no tie-break reduction; ties go
+        // to the largest row position
+        const unsigned long long cand = __ballot(key == kmax && !done);
+        int pv;
+        if (__builtin_popcountll(cand) == 1) {
+            pv = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(cand));
+        } else {
+            const int Pt = wave_max_i((key == kmax && !done) ? pos : -1);
+            pv = __builtin_amdgcn_readfirstlane(
+                (int)__builtin_ctzll(__ballot(pos == Pt && !done)));
+        }
+        const int P = __builtin_amdgcn_readlane(pos, pv);
         if (lane == j) plv = pv;
         if (pos == j) pos = P;
         if (lane == pv) {

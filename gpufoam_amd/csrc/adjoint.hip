@@ -114,6 +114,7 @@ adjoint_reverse_kernel(SolveArgs A, double* scratch, long stride, long laneD, lo
     adj::AdjBuf a{b, scratch + laneD * stride + cell, stride, n};
     a.b.raw = a.mat(1);
     a.b.jt = a.mat(2);
+    a.b.f2 = a.f2().p;
     const double p = y0copy[(long)(nS + 1) * stride + cell];
     const double T0 = y0copy[(long)nS * stride + cell];
     lane::SV yb = a.ybar();

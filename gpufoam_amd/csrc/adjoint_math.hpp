@@ -66,7 +66,8 @@ struct AdjBuf {
     GFC_HD __forceinline__ SV vb() const { return vec(15); }
     GFC_HD __forceinline__ SV dNb() const { return vec(16); }
     GFC_HD __forceinline__ SV dCp() const { return vec(17); }
-    static constexpr int kVectors = 18;
+    GFC_HD __forceinline__ SV f2() const { return vec(18); }
+    static constexpr int kVectors = 19;
     GFC_HD __forceinline__ double* mat(int k) const {
         return base + ((long)kVectors * n + (long)k * n * n) * s;
     }
@@ -409,13 +410,15 @@ GFC_HD inline __noinline__ void reaction_reverse(const DeviceMech& m, int r, dou
 
 // ---------------------------------------------------- right-hand side --
 
-// ybar += d<fbar, f(y)>/dy, th += d<fbar, f(y)>/dtheta.
+// ybar += d<fbar, f(y)>/dy, th += d<fbar, f(y)>/dtheta.  fy: f(y) when the
+// caller has it (fy.p non-null), else it is evaluated.
 GFC_HD inline __noinline__ void vjp_f(const DeviceMech& m, double p, SV y, SV fbar, SV ybar,
-                               ThView th, const AdjBuf& a) {
+                               ThView th, const AdjBuf& a, SV fy) {
     const int nS = m.nS;
     const LaneBuf& b = a.b;
-    SV f = a.fval(), c = b.c(), Ha = b.Ha(), Cp = b.Cp(), dCp = a.dCp();
-    lane::derivatives(m, p, y, f, b);
+    SV f = fy.p ? fy : a.fval();
+    SV c = b.c(), Ha = b.Ha(), Cp = b.Cp(), dCp = a.dCp();
+    if (!fy.p) lane::derivatives(m, p, y, f, b);
     const Mix x = mixture(m, p, y, c, Ha, Cp, dCp);
     SV cb = a.cb(), Hab = a.Hab(), dNb = a.dNb();
     MixBar mb;
@@ -651,7 +654,7 @@ GFC_HD inline __noinline__ void step_reverse(const SolveArgs& A, double p, SV y0
             ys[i] = y0[i] + t.a31 * k1[i] + t.a32 * k2[i];
             ysb[i] = 0.0;
         }
-        vjp_f(m, p, ys, fb3, ysb, th, a);
+        vjp_f(m, p, ys, fb3, ysb, th, a, b.dydx()); // the step left f(y3) there
         for (int i = 0; i < n; ++i) {
             yb[i] += ysb[i];
             kb1[i] += t.a31 * ysb[i];
@@ -670,7 +673,7 @@ GFC_HD inline __noinline__ void step_reverse(const SolveArgs& A, double p, SV y0
         ys[i] = y0[i] + t.a21 * k1[i];
         ysb[i] = 0.0;
     }
-    vjp_f(m, p, ys, fb2, ysb, th, a);
+    vjp_f(m, p, ys, fb2, ysb, th, a, t.stages == 4 ? a.f2() : b.dydx());
     for (int i = 0; i < n; ++i) {
         yb[i] += ysb[i];
         kb1[i] += t.a21 * ysb[i];
@@ -679,8 +682,16 @@ GFC_HD inline __noinline__ void step_reverse(const SolveArgs& A, double p, SV y0
     lu_solve_T(b, rb);
     add_outer(rb, k1);
     for (int i = 0; i < n; ++i) Fb[i] += h * t.d1 * rb[i];
-    // rb is now the adjoint of f(y0) (the dydx0 of the step)
-    vjp_f(m, p, y0, rb, yb, th, a);
+    // rb is now the adjoint of f(y0) (the dydx0 of the step).  Within the
+    // reaction temperature limits f(y0) is the same function as the
+    // Jacobian call's F (omega clips T, the Jacobian does not), so its
+    // adjoint joins Fbar and one reverse sweep serves both.
+    const double T0 = y0[m.nS];
+    if (T0 >= m.Tlow && T0 <= m.Thigh) {
+        for (int i = 0; i < n; ++i) Fb[i] += rb[i];
+    } else {
+        vjp_f(m, p, y0, rb, yb, th, a, b.dydx0());
+    }
     jact(m, p, y0, yb, th, a);
 }
 

@@ -44,6 +44,9 @@ struct LaneBuf {
     // transforms, `jt` the transformed Jacobian before the LU overwrites it.
     double* raw = nullptr;
     double* jt = nullptr;
+    // optional copy of the first stage's right-hand side f(y2), which the
+    // step overwrites with f(y3)
+    double* f2 = nullptr;
     GFC_HD __forceinline__ SV at(int k) const { return SV{base + (long)k * n * s, s}; }
     GFC_HD __forceinline__ SV y() const { return at(0); }
     GFC_HD __forceinline__ SV dydx0() const { return at(1); }
@@ -416,6 +419,8 @@ GFC_HD inline double ros_step(const SolveArgs& A, double p, SV y0, double dx,
     lu_backsub(b, k1);
     for (int i = 0; i < n; ++i) y[i] = y0[i] + t.a21 * k1[i];
     derivatives(A.m, p, y, dydx, b);
+    if (b.f2)
+        for (int i = 0; i < n; ++i) b.f2[(long)i * b.s] = dydx[i];
     for (int i = 0; i < n; ++i) k2[i] = dydx[i] + dx * t.d2 * dfdx[i] + t.c21 * k1[i] / dx;
     lu_backsub(b, k2);
     double maxErr = 0.0;

@@ -60,6 +60,7 @@ struct Work {
         a = adj::AdjBuf{b, adj.data(), 1, n};
         a.b.raw = a.mat(1);
         a.b.jt = a.mat(2);
+        a.b.f2 = a.f2().p;
     }
 };
 
@@ -95,7 +96,7 @@ void adjh_vjp_f(const gfc_mech_view* v, double p, const double* y, const double*
     const int n = w.n;
     std::vector<double> yy(y, y + n), fb(fbar, fbar + n);
     adj::vjp_f(hm.d, p, lane::SV{yy.data(), 1}, lane::SV{fb.data(), 1}, lane::SV{ybar, 1},
-               adj::ThView{th, 1}, w.a);
+               adj::ThView{th, 1}, w.a, lane::SV{nullptr, 1});
 }
 
 // ybar += d/dy, th += d/dparams of <Jbar, J(y)> + <Fbar, F(y)> (J n x n).
@@ -205,7 +206,7 @@ void adjh_replay(const gfc_mech_view* v, const gfc_ode_inputs* ode, double* y, i
     const SolveArgs A = args_for(hm.d, ode);
     const double p = y[nS + 1];
     lane::LaneBuf b = w.a.b;
-    b.raw = b.jt = nullptr;
+    b.raw = b.jt = b.f2 = nullptr;
     copy_in(b.y(), y, n);
     for (int t = 0; t < N; ++t) {
         lane::derivatives(hm.d, p, b.y(), b.dydx0(), b);

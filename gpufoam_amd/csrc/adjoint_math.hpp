@@ -299,7 +299,7 @@ struct RxnBar {
 // Reverse of reaction r at temperature T (clipped to [Tlow, Thigh] when
 // `clip`, as gpuReaction::omega does; the Jacobian uses T unclipped).
 // Adds to cbar, Tb and th[0..5].
-GFC_HD inline __noinline__ void reaction_reverse(const DeviceMech& m, int r, double T, bool clip,
+GFC_HD __forceinline__ void reaction_reverse(const DeviceMech& m, int r, double T, bool clip,
                                           SV c, const RxnBar& in, SV cbar, double& Tb,
                                           double* th) {
     const double Te = clip ? fmin(fmax(T, m.Tlow), m.Thigh) : T;

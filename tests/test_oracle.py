@@ -350,3 +350,32 @@ def test_rosenbrock_order(solver, order):
         errs.append(np.abs(y[:nS] - yr[:nS]).max())
     for a, b in zip(errs[:-1], errs[1:]):
         assert 0.6 * 2 ** (order + 1) < a / b < 1.5 * 2 ** (order + 1), errs
+
+
+def test_counted_work_constants_are_reproducible():
+    """gpufoam_amd/flops/<mech>.json (operation counts of the oracle) match
+    a fresh count, and the LU / solve counts match their closed forms."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    if not os.path.exists(os.path.join(root, "oracle", "libopcount.so")):
+        pytest.skip("oracle/libopcount.so not built")
+    committed = {}
+    for mech in ("h2", "gri", "syn200"):
+        with open(os.path.join(root, "gpufoam_amd", "flops", f"{mech}.json")) as f:
+            committed[mech] = json.load(f)
+    for mech, d in committed.items():
+        n = d["n"]
+        # Crout LU: n(n-1)(2n-1)/6 * 2 products and sums, n divisions of the
+        # row scale, n(n-1)/2 column scalings
+        assert abs(d["lu"] - 2 * n ** 3 / 3) <= 1.5 * n * n + 10 * n
+        assert abs(d["solve"] - 2 * n * n) <= 2 * n
+        assert d["attempt"] == d["F_jac"] + d["lu"] + 4 * d["solve"] + 2 * d["F_rhs"] + 40 * n
+    out = subprocess.run([sys.executable, os.path.join(root, "oracle", "count_flops.py")],
+                         capture_output=True, text=True, check=True).stdout
+    for mech, d in committed.items():
+        with open(os.path.join(root, "gpufoam_amd", "flops", f"{mech}.json")) as f:
+            assert json.load(f) == d, mech
+    assert "syn200" in out

@@ -604,8 +604,14 @@ GFC_HD inline __noinline__ void step_reverse(const SolveArgs& A, double p, SV y0
     const int n = a.n;
     const LaneBuf& b = a.b;
     const RosTableau& t = A.tab;
-    lane::derivatives(m, p, y0, b.dydx0(), b);
-    (void)lane::ros_step(A, p, y0, h, b.yTemp(), b);
+    // recompute the step; within the reaction temperature limits f(y0) is
+    // the Jacobian call's F, so the separate right-hand side is skipped
+    const double T0 = y0[m.nS];
+    const bool inRange = T0 >= m.Tlow && T0 <= m.Thigh;
+    LaneBuf bs = b;
+    bs.dydx0FromF = inRange;
+    if (!inRange) lane::derivatives(m, p, y0, b.dydx0(), b);
+    (void)lane::ros_step(A, p, y0, h, b.yTemp(), bs);
     SV yb = a.ybar(), rb = a.rb(), Fb = a.Fb(), fb2 = a.fb2(), fb3 = a.fb3();
     SV k1 = b.k1(), k2 = b.k2(), k3 = b.k3(), k4 = b.k4();
     SV kb1 = a.kb(0), kb2 = a.kb(1), kb3 = a.kb(2), kb4 = a.kb(3);
@@ -686,8 +692,7 @@ GFC_HD inline __noinline__ void step_reverse(const SolveArgs& A, double p, SV y0
     // reaction temperature limits f(y0) is the same function as the
     // Jacobian call's F (omega clips T, the Jacobian does not), so its
     // adjoint joins Fbar and one reverse sweep serves both.
-    const double T0 = y0[m.nS];
-    if (T0 >= m.Tlow && T0 <= m.Thigh) {
+    if (inRange) {
         for (int i = 0; i < n; ++i) Fb[i] += rb[i];
     } else {
         vjp_f(m, p, y0, rb, yb, th, a, b.dydx0());

@@ -47,6 +47,9 @@ struct LaneBuf {
     // optional copy of the first stage's right-hand side f(y2), which the
     // step overwrites with f(y3)
     double* f2 = nullptr;
+    // take the step's f(y0) from the Jacobian call's F instead of the caller
+    // (the same function of y0 within the reaction temperature limits)
+    bool dydx0FromF = false;
     GFC_HD __forceinline__ SV at(int k) const { return SV{base + (long)k * n * s, s}; }
     GFC_HD __forceinline__ SV y() const { return at(0); }
     GFC_HD __forceinline__ SV dydx0() const { return at(1); }
@@ -407,6 +410,8 @@ GFC_HD inline double ros_step(const SolveArgs& A, double p, SV y0, double dx,
     jacobian(A.m, p, y0, b);
     if (b.jt)
         for (int i = 0; i < n * n; ++i) b.jt[(long)i * b.s] = b.J()[i];
+    if (b.dydx0FromF)
+        for (int i = 0; i < n; ++i) b.dydx0()[i] = b.dfdx()[i];
     // computeDecomposedJacobian (gpuAdaptiveSolver.H:51-73)
     const double temp = 1.0 / (t.gamma * dx);
     for (int i = 0; i < n; i++) {

@@ -41,6 +41,7 @@ bool hipFail(hipError_t e, const char* what) {
 
 struct AdjState {
     long cap = 0;   // cells of workspace (multiple of 64)
+    long maxSteps = 0;
     int K = 0;      // checkpoint interval in use
     int Kreq = 0;   // requested interval (0: automatic)
     long laneD = 0, adjD = 0;
@@ -71,22 +72,37 @@ AdjState* state(gfc_evaluator* ev) {
 
 } // namespace
 
+// Sizes of the per-cell adjoint workspace.  Every per-cell array is
+// wave-blocked (lane::wave_slot): the 64 cells of a wavefront share one
+// contiguous block, element e of a cell at [e*64] from its slot.
+struct AdjLayout {
+    long laneD, adjD;   // forward and adjoint scratch doubles per cell
+    long perScratch;    // laneD + adjD + K*n
+    long nck;           // checkpoint slots per cell
+    long maxSteps;      // recorded step sizes per cell
+    int K;              // checkpoint interval
+};
+
 // Forward solve with recording (one lane per cell).
 __global__ void __launch_bounds__(64)
-adjoint_forward_kernel(SolveArgs A, double* scratch, long stride, double* ck,
-                       double* hs, int K, int* nAcc, double* yEnd, double* y0copy) {
+adjoint_forward_kernel(SolveArgs A, AdjLayout W, double* scratch, double* ck, double* hs,
+                       int* nAcc, double* yEnd, double* y0copy) {
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= A.nCells) return;
     const int nS = A.m.nS, n = nS + 1;
-    const lane::LaneBuf b{scratch + cell, stride, n};
+    constexpr long s = lane::kSlotStride;
+    const lane::LaneBuf b{lane::wave_slot(scratch, cell, W.perScratch), s, n};
+    double* y0c = lane::wave_slot(y0copy, cell, n + 1);
     const long ld = A.ldY;
-    for (int k = 0; k < nS + 2; ++k) y0copy[k * stride + cell] = A.Y0[k * ld + cell];
+    for (int k = 0; k < nS + 2; ++k) y0c[k * s] = A.Y0[k * ld + cell];
     for (int k = 0; k < n; ++k) b.y()[k] = A.Y0[k * ld + cell];
     const double p = A.Y0[(long)(nS + 1) * ld + cell];
     double dtChem = A.dtChem[cell];
     int attempts = 0, accepted = 0;
-    adj::Recorder rec{ck + cell, hs + cell, stride, n, K, 0};
-    const int status = adj::cell_forward(A, p, b, dtChem, rec, lane::SV{yEnd + cell, stride},
+    adj::Recorder rec{lane::wave_slot(ck, cell, W.nck * n), lane::wave_slot(hs, cell, W.maxSteps),
+                      s, n, W.K, 0};
+    const int status = adj::cell_forward(A, p, b, dtChem, rec,
+                                         lane::SV{lane::wave_slot(yEnd, cell, n), s},
                                          attempts, accepted);
     nAcc[cell] = rec.count;
     if (A.Ynew) {
@@ -103,47 +119,52 @@ adjoint_forward_kernel(SolveArgs A, double* scratch, long stride, double* ck,
 
 // Reverse sweep (one lane per cell).  Ybar/Y0bar: [n][ldb]; th: [6 nR][ldb].
 __global__ void __launch_bounds__(64)
-adjoint_reverse_kernel(SolveArgs A, double* scratch, long stride, long laneD, long adjD,
-                       const double* ck, const double* hs, int K, const int* nAcc,
-                       double* yEnd, const double* y0copy, const double* Ybar,
+adjoint_reverse_kernel(SolveArgs A, AdjLayout W, double* scratch, double* ck, double* hs,
+                       const int* nAcc, double* yEnd, double* y0copy, const double* Ybar,
                        long ldb, double* Y0bar, double* thBar) {
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= A.nCells) return;
     const int nS = A.m.nS, n = nS + 1;
-    lane::LaneBuf b{scratch + cell, stride, n};
-    adj::AdjBuf a{b, scratch + laneD * stride + cell, stride, n};
+    constexpr long s = lane::kSlotStride;
+    double* slot = lane::wave_slot(scratch, cell, W.perScratch);
+    lane::LaneBuf b{slot, s, n};
+    adj::AdjBuf a{b, slot + W.laneD * s, s, n};
     a.b.raw = a.mat(1);
     a.b.jt = a.mat(2);
     a.b.f2 = a.f2().p;
-    const double p = y0copy[(long)(nS + 1) * stride + cell];
-    const double T0 = y0copy[(long)nS * stride + cell];
+    const double* y0c = lane::wave_slot(y0copy, cell, n + 1);
+    const double p = y0c[(long)(nS + 1) * s];
+    const double T0 = y0c[(long)nS * s];
     lane::SV yb = a.ybar();
     for (int e = 0; e < n; ++e) yb[e] = Ybar[e * ldb + cell];
     const adj::ThView th{thBar ? thBar + cell : nullptr, ldb};
     if (th.p)
         for (long q = 0; q < (long)adj::kParams * A.m.nR; ++q) th.p[q * ldb] = 0.0;
-    adj::cell_reverse(A, p, T0, a, scratch + (laneD + adjD) * stride + cell, ck + cell,
-                      hs + cell, stride, K, nAcc[cell], lane::SV{yEnd + cell, stride}, th);
+    adj::cell_reverse(A, p, T0, a, slot + (W.laneD + W.adjD) * s,
+                      lane::wave_slot(ck, cell, W.nck * n), lane::wave_slot(hs, cell, W.maxSteps),
+                      s, W.K, nAcc[cell], lane::SV{lane::wave_slot(yEnd, cell, n), s}, th);
     for (int e = 0; e < n; ++e) Y0bar[e * ldb + cell] = yb[e];
 }
 
 // Re-integrates the recorded accepted steps from Y0 (one lane per cell).
 __global__ void __launch_bounds__(64)
-adjoint_replay_kernel(SolveArgs A, double* scratch, long stride, const double* hs,
-                      const int* nAcc, const double* y0copy) {
+adjoint_replay_kernel(SolveArgs A, AdjLayout W, double* scratch, double* hs, const int* nAcc,
+                      double* y0copy) {
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= A.nCells) return;
     const int nS = A.m.nS, n = nS + 1;
-    const lane::LaneBuf b{scratch + cell, stride, n};
+    constexpr long s = lane::kSlotStride;
+    const lane::LaneBuf b{lane::wave_slot(scratch, cell, W.perScratch), s, n};
+    const double* hc = lane::wave_slot(hs, cell, W.maxSteps);
     const long ld = A.ldY;
     for (int k = 0; k < n; ++k) b.y()[k] = A.Y0[k * ld + cell];
     const double p = A.Y0[(long)(nS + 1) * ld + cell];
-    const double T0 = y0copy[(long)nS * stride + cell];
+    const double T0 = lane::wave_slot(y0copy, cell, n + 1)[(long)nS * s];
     if (T0 > A.Treact && A.deltaT > kSmall) {
         const int N = nAcc[cell];
         for (int t = 0; t < N; ++t) {
             lane::derivatives(A.m, p, b.y(), b.dydx0(), b);
-            (void)lane::ros_step(A, p, b.y(), hs[(long)t * stride + cell], b.yTemp(), b);
+            (void)lane::ros_step(A, p, b.y(), hc[(long)t * s], b.yTemp(), b);
             for (int k = 0; k < n; ++k) b.y()[k] = b.yTemp()[k];
         }
         for (int i = 0; i < nS; i++) b.y()[i] = fmax(0.0, b.y()[i]);
@@ -158,6 +179,19 @@ void adjoint_free(gfc_evaluator* ev) {
     st->release();
     delete st;
     ev->adjoint = nullptr;
+}
+
+static AdjLayout layout(gfc_evaluator* ev) {
+    AdjState* st = state(ev);
+    const long n = ev->nS + 1;
+    AdjLayout W;
+    W.laneD = st->laneD;
+    W.adjD = st->adjD;
+    W.K = st->K;
+    W.perScratch = st->laneD + st->adjD + (long)st->K * n;
+    W.nck = (st->maxSteps + st->K - 1) / st->K;
+    W.maxSteps = st->maxSteps;
+    return W;
 }
 
 // Sizes (and allocates) the workspace for nCells cells.
@@ -184,6 +218,7 @@ static int adjoint_reserve(gfc_evaluator* ev, int nCells) {
     st->adjD = adj::adj_scratch_doubles(ev->nS);
     st->K = K;
     st->cap = cap;
+    st->maxSteps = maxSteps;
     const long nck = (maxSteps + K - 1) / K;
     const size_t scratchB = (size_t)8 * (st->laneD + st->adjD + (long)K * n) * cap;
     if (hipFail(hipMalloc(&st->scratch, scratchB), "hipMalloc(adjoint scratch)") ||
@@ -294,8 +329,8 @@ int gfc_adjoint_forward(gfc_evaluator* ev, double deltaT, int nCells, int ldY,
     A.dtChem = deltaTChem;
     A.stats = cellStats;
     const int blocks = (nCells + 63) / 64;
-    hipLaunchKernelGGL(adjoint_forward_kernel, dim3(blocks), dim3(64), 0, s, A, st->scratch,
-                       st->cap, st->ck, st->hs, st->K, st->nAcc, st->yEnd, st->y0);
+    hipLaunchKernelGGL(adjoint_forward_kernel, dim3(blocks), dim3(64), 0, s, A, layout(ev),
+                       st->scratch, st->ck, st->hs, st->nAcc, st->yEnd, st->y0);
     if (hipFail(hipGetLastError(), "adjoint forward launch")) return GFC_ERR_HIP;
     st->nCells = nCells;
     st->deltaT = deltaT;
@@ -318,9 +353,9 @@ int gfc_adjoint_backward(gfc_evaluator* ev, const double* Ybar, int ldb, double*
     hipStream_t s = static_cast<hipStream_t>(stream);
     SolveArgs A = solve_args(ev, st->deltaT, 1e300, st->nCells);
     const int blocks = (st->nCells + 63) / 64;
-    hipLaunchKernelGGL(adjoint_reverse_kernel, dim3(blocks), dim3(64), 0, s, A, st->scratch,
-                       st->cap, st->laneD, st->adjD, st->ck, st->hs, st->K, st->nAcc,
-                       st->yEnd, st->y0, Ybar, (long)ldb, Y0bar, paramBar);
+    hipLaunchKernelGGL(adjoint_reverse_kernel, dim3(blocks), dim3(64), 0, s, A, layout(ev),
+                       st->scratch, st->ck, st->hs, st->nAcc, st->yEnd, st->y0, Ybar,
+                       (long)ldb, Y0bar, paramBar);
     if (hipFail(hipGetLastError(), "adjoint reverse launch")) return GFC_ERR_HIP;
     return GFC_OK;
 }
@@ -343,8 +378,8 @@ int gfc_adjoint_replay(gfc_evaluator* ev, const double* Y0, int ldY, double* Yne
     A.Y0 = Y0;
     A.Ynew = Ynew;
     const int blocks = (st->nCells + 63) / 64;
-    hipLaunchKernelGGL(adjoint_replay_kernel, dim3(blocks), dim3(64), 0, s, A, st->scratch,
-                       st->cap, st->hs, st->nAcc, st->y0);
+    hipLaunchKernelGGL(adjoint_replay_kernel, dim3(blocks), dim3(64), 0, s, A, layout(ev),
+                       st->scratch, st->hs, st->nAcc, st->y0);
     if (hipFail(hipGetLastError(), "adjoint replay launch")) return GFC_ERR_HIP;
     return GFC_OK;
 }

@@ -23,7 +23,7 @@ namespace gfc {
 
 template <int N>
 __global__ void team_solve_kernel(SolveArgs A, int* queue);
-__global__ void lane_solve_kernel(SolveArgs A, double* scratch, long stride);
+__global__ void lane_solve_kernel(SolveArgs A, double* scratch, long per);
 
 namespace {
 
@@ -115,7 +115,7 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
         // one lane per cell
         const int blocks = (A.nCells + 63) / 64;
         hipLaunchKernelGGL(lane_solve_kernel, dim3(blocks), dim3(64), 0, s, A,
-                           ev->laneScratch, ev->laneStride);
+                           ev->laneScratch, ev->lanePer);
         return hipFail(hipGetLastError(), "lane launch") ? GFC_ERR_HIP : GFC_OK;
     }
     int N = 0;
@@ -124,7 +124,7 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
         // mechanism outside the wavefront kernel's capacity: one lane per cell
         const int blocks = (A.nCells + 63) / 64;
         hipLaunchKernelGGL(lane_solve_kernel, dim3(blocks), dim3(64), 0, s, A,
-                           ev->laneScratch, ev->laneStride);
+                           ev->laneScratch, ev->lanePer);
         return hipFail(hipGetLastError(), "lane launch") ? GFC_ERR_HIP : GFC_OK;
     }
     SolveArgs At = A;

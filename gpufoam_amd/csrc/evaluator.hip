@@ -18,7 +18,7 @@
 
 namespace gfc {
 
-__global__ void lane_solve_kernel(SolveArgs A, double* scratch, long stride);
+__global__ void lane_solve_kernel(SolveArgs A, double* scratch, long per);
 long lane_scratch_per_cell(int nS);
 
 namespace {
@@ -305,7 +305,8 @@ int gfc_evaluator_create(const gfc_mechanism* mech, const gfc_ode_inputs* ode,
     }
     // plain kernel scratch: one slot per cell, rounded to whole wavefronts
     ev->laneStride = ((long)maxCells + 63) / 64 * 64;
-    ev->laneScratchBytes = sizeof(double) * lane_scratch_per_cell(ev->nS) * ev->laneStride;
+    ev->lanePer = lane_scratch_per_cell(ev->nS);
+    ev->laneScratchBytes = sizeof(double) * ev->lanePer * ev->laneStride;
     const int nE = ev->nS + 2;
     const size_t ld = ev->laneStride;
     if (hipFail(hipMalloc(&ev->laneScratch, ev->laneScratchBytes), "hipMalloc(scratch)") ||
@@ -414,7 +415,7 @@ int gfc_solve(gfc_evaluator* ev, double deltaT, double deltaTChemMax,
     } else {
         const int blocks = (nCells + 63) / 64;
         hipLaunchKernelGGL(lane_solve_kernel, dim3(blocks), dim3(64), 0, s, A,
-                           ev->laneScratch, ev->laneStride);
+                           ev->laneScratch, ev->lanePer);
     }
     if (hipFail(hipGetLastError(), "kernel launch")) return GFC_ERR_HIP;
     if (deltaTMin)

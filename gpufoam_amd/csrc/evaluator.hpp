@@ -20,7 +20,8 @@ struct gfc_evaluator {
     void* mechBlockTeam = nullptr;
     // plain kernel: per-cell SoA scratch
     double* laneScratch = nullptr;
-    long laneStride = 0;
+    long laneStride = 0;  // cells of scratch (maxCells rounded to 64)
+    long lanePer = 0;     // scratch doubles per cell (wave-blocked layout)
     size_t laneScratchBytes = 0;
     unsigned long long* minBits = nullptr;
     // staging for GFC_HOST_POINTERS

@@ -25,11 +25,11 @@ using namespace lane;
 
 // singleCellSolver + computeRR epilogue for one cell in one lane.
 __global__ void __launch_bounds__(64, 4)
-lane_solve_kernel(SolveArgs A, double* scratch, long stride) {
+lane_solve_kernel(SolveArgs A, double* scratch, long per) {
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= A.nCells) return;
     const int nS = A.m.nS, n = nS + 1;
-    const LaneBuf b{scratch + cell, stride, nS + 1};
+    const LaneBuf b{wave_slot(scratch, cell, per), kSlotStride, nS + 1};
     const long ld = A.ldY;
     for (int k = 0; k < n; ++k) b.y()[k] = A.Y0[k * ld + cell];
     const double p = A.Y0[(long)(nS + 1) * ld + cell];

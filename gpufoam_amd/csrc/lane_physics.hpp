@@ -74,6 +74,16 @@ struct LaneBuf {
     }
 };
 
+// Per-cell scratch is laid out wave-blocked: the 64 cells of a wavefront
+// share one contiguous block of per*64 doubles, element e of cell c at
+// [e*64 + c%64] of block c/64, so every access of a wavefront is one
+// coalesced 512-byte line and a wavefront's whole scratch is one compact
+// region (few pages, good L2 reuse).
+constexpr long kSlotStride = 64;
+GFC_HD __forceinline__ double* wave_slot(double* base, long cell, long per) {
+    return base + (cell / kSlotStride) * (per * kSlotStride) + (cell % kSlotStride);
+}
+
 // Number of doubles of scratch per slot for nS species.
 GFC_HD inline long lane_scratch_doubles(int nS) {
     const long n = nS + 1;

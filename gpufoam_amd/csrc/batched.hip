@@ -23,6 +23,8 @@ namespace gfc {
 
 template <int N>
 __global__ void team_solve_kernel(SolveArgs A, int* queue);
+template <int NA, int NC>
+__global__ void block_solve_kernel(SolveArgs A, int* queue, double* scratch, long perWG);
 __global__ void lane_solve_kernel(SolveArgs A, double* scratch, long per);
 
 namespace {
@@ -39,9 +41,32 @@ struct BatchedState {
     int lastGrid = 0;
     int lastN = 0;
     bool valid = false;
+    // workgroup-per-cell kernel (mechanisms with 64 < n <= kBlockMaxN)
+    int blkNA = 0;             // rows per thread (0: not available)
+    int blkGrid = 0;           // resident workgroups
+    long blkPerWG = 0;         // scratch doubles per workgroup
+    double* blkScratch = nullptr;
 };
 
 using KernelFn = void (*)(SolveArgs, int*);
+using BlockFn = void (*)(SolveArgs, int*, double*, long);
+
+constexpr int kBlockThreads = 512;
+
+BlockFn block_kernel_for(int NA) {
+    if (NA == 8) return block_solve_kernel<8, 4>;
+    if (NA == 13) return block_solve_kernel<13, 7>;
+    return nullptr;
+}
+
+// One launch of the batched form: the wavefront kernel or the workgroup
+// kernel, with its persistent grid.
+struct Plan {
+    KernelFn team = nullptr;
+    BlockFn blk = nullptr;
+    int grid = 0;
+    int key = 0; // identifies the kernel for graph reuse
+};
 
 // Row capacity of the wavefront kernel for n = nSpecie+1: exact sizes for
 // the bundled mechanisms, padded capacities otherwise.
@@ -70,6 +95,26 @@ int batched_init(gfc_evaluator* ev) {
     if (hipFail(hipGetDeviceProperties(&prop, ev->device), "hipGetDeviceProperties"))
         return GFC_ERR_HIP;
     b->numCUs = prop.multiProcessorCount;
+    // workgroup-per-cell kernel: scratch for its resident grid
+    const int n = ev->nS + 1;
+    if (block_eligible(ev->mech.view()) && ev->dmTeam.nSets <= kBlockMaxSets) {
+        const int NA = n <= 128 ? 8 : 13;
+        const int NC = NA == 8 ? 4 : 7;
+        int occ = 0;
+        if (hipFail(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &occ, (const void*)block_kernel_for(NA), kBlockThreads, 0),
+                    "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
+            return GFC_ERR_HIP;
+        if (occ > 0) {
+            const long nRp = ((long)ev->nR + 63) / 64 * 64;
+            b->blkNA = NA;
+            b->blkGrid = b->numCUs * occ;
+            b->blkPerWG = 8 * nRp + 2L * NA * NC * kBlockThreads;
+            if (hipFail(hipMalloc(&b->blkScratch, sizeof(double) * b->blkPerWG * b->blkGrid),
+                        "hipMalloc(workgroup scratch)"))
+                return GFC_ERR_HIP;
+        }
+    }
     // Graphs are recorded on a private stream and launched on the caller's:
     // the caller's stream may be the null stream, which cannot be captured.
     if (hipFail(hipStreamCreateWithFlags(&b->captureStream, hipStreamNonBlocking),
@@ -84,6 +129,7 @@ void batched_free(gfc_evaluator* ev) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->queue) (void)hipFree(b->queue);
+    if (b->blkScratch) (void)hipFree(b->blkScratch);
     if (b->captureStream) (void)hipStreamDestroy(b->captureStream);
     delete b;
     ev->batched = nullptr;
@@ -99,10 +145,13 @@ static int resident_blocks(KernelFn fn) {
     return blocks;
 }
 
-static int enqueue(gfc_evaluator* ev, BatchedState* b, const SolveArgs& A,
-                   hipStream_t s, KernelFn fn, int grid) {
+static int enqueue(BatchedState* b, const SolveArgs& A, hipStream_t s, const Plan& pl) {
     hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, s, b->queue);
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(64), 0, s, A, b->queue);
+    if (pl.blk)
+        hipLaunchKernelGGL(pl.blk, dim3(pl.grid), dim3(kBlockThreads), 0, s, A, b->queue,
+                           b->blkScratch, b->blkPerWG);
+    else
+        hipLaunchKernelGGL(pl.team, dim3(pl.grid), dim3(64), 0, s, A, b->queue);
     if (hipFail(hipGetLastError(), "persistent launch")) return GFC_ERR_HIP;
     return GFC_OK;
 }
@@ -119,9 +168,20 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
         return hipFail(hipGetLastError(), "lane launch") ? GFC_ERR_HIP : GFC_OK;
     }
     int N = 0;
-    KernelFn fn = team_kernel_for(n, N);
-    if (!fn || ev->dmTeam.teamWidth > kTeamWidth) {
-        // mechanism outside the wavefront kernel's capacity: one lane per cell
+    Plan pl;
+    pl.team = team_kernel_for(n, N);
+    if (pl.team && ev->dmTeam.teamWidth <= kTeamWidth) {
+        int perCU = resident_blocks(pl.team);
+        if (ev->wavesPerBlock > 0 && ev->wavesPerBlock < perCU) perCU = ev->wavesPerBlock;
+        pl.grid = b->numCUs * perCU;
+        pl.key = N;
+    } else if (b->blkNA > 0) {
+        pl.team = nullptr;
+        pl.blk = block_kernel_for(b->blkNA);
+        pl.grid = b->blkGrid;
+        pl.key = 1000 + b->blkNA;
+    } else {
+        // mechanism outside both kernels' capacity: one lane per cell
         const int blocks = (A.nCells + 63) / 64;
         hipLaunchKernelGGL(lane_solve_kernel, dim3(blocks), dim3(64), 0, s, A,
                            ev->laneScratch, ev->lanePer);
@@ -129,10 +189,8 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
     }
     SolveArgs At = A;
     At.m = ev->dmTeam;
-    int perCU = resident_blocks(fn);
-    if (ev->wavesPerBlock > 0 && ev->wavesPerBlock < perCU) perCU = ev->wavesPerBlock;
-    int grid = b->numCUs * perCU;
-    if (grid > A.nCells) grid = A.nCells > 0 ? A.nCells : 1;
+    if (pl.grid > A.nCells) pl.grid = A.nCells > 0 ? A.nCells : 1;
+    const int grid = pl.grid;
     if (graph) {
         // inside a caller's own capture the launches simply join that graph
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -140,10 +198,10 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
             cs != hipStreamCaptureStatusNone)
             graph = false;
     }
-    if (!graph) return enqueue(ev, b, At, s, fn, grid);
+    if (!graph) return enqueue(b, At, s, pl);
 
     // graph path: reuse the executable graph when nothing changed
-    const bool same = b->valid && b->lastGrid == grid && b->lastN == N &&
+    const bool same = b->valid && b->lastGrid == grid && b->lastN == pl.key &&
                       std::memcmp(&b->lastArgs, &At, sizeof(SolveArgs)) == 0;
     if (!same) {
         hipGraph_t g = nullptr;
@@ -151,7 +209,7 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
         if (hipFail(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal),
                     "hipStreamBeginCapture"))
             return GFC_ERR_HIP;
-        int st = enqueue(ev, b, At, cs, fn, grid);
+        int st = enqueue(b, At, cs, pl);
         hipError_t e = hipStreamEndCapture(cs, &g);
         if (st != GFC_OK) return st;
         if (hipFail(e, "hipStreamEndCapture")) return GFC_ERR_HIP;
@@ -176,7 +234,7 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
         b->graph = g;
         b->lastArgs = At;
         b->lastGrid = grid;
-        b->lastN = N;
+        b->lastN = pl.key;
         b->valid = true;
     }
     if (hipFail(hipGraphLaunch(b->exec, s), "hipGraphLaunch")) return GFC_ERR_HIP;

@@ -13,6 +13,36 @@ namespace gfc {
 constexpr int kWidth = 10; // gpuReaction.H:12 MAX_REACTION_WIDTH
 constexpr int kNoIntExp = 214748364;
 
+// Gather lists of the workgroup-per-cell kernel (reaction indices in the
+// stored order).  Empty unless that kernel can run the mechanism.
+struct BlockTables {
+    // reactions of each specie with its net stoichiometric coefficient:
+    // dN_i/dt = sum_e spNu[e] w[spR[e]], e in [spPtr[i], spPtr[i+1])
+    const int* spPtr;
+    const int* spR;
+    const double* spNu;
+    // contributions to the concentration Jacobian entry (i, j), row-major
+    // [nS*nS+1]: J(i,j) = sum_e elNu[e] D[elRS[e] & 7][elRS[e] >> 3], where
+    // D[q][r] is dw_r/dc of the specie at position q of reaction r
+    // (0-2 left side, 3-5 right side)
+    const int* elPtr;
+    const int* elRS;
+    const double* elNu;
+    // the same contributions by row and 32-column block, sorted by column
+    // then reaction: rbPtr[i*nCB + b] .. rbPtr[i*nCB + b + 1] (nCB = column
+    // blocks of the specie columns), rbEnt = (column in block << 24) | (r*8 + q)
+    const int* rbPtr;
+    const int* rbEnt;
+    const double* rbNu;
+    // third-body reactions of each specie row: J(i,j) += tbNu g_r effB[set][j]
+    const int* tbPtr;
+    const int* tbR;
+    const int* tbSet;
+    const double* tbNu;
+    const double* effB; // [nSets*nS] efficiency sets, set-major
+    const double* effT; // [nS*nSets] specie-major
+};
+
 // Flat device copy of gfc_mech_view plus host-precomputed constants.  Every
 // array is in one hipMalloc'd block owned by the evaluator.
 struct DeviceMech {
@@ -70,6 +100,7 @@ struct DeviceMech {
     const double* recD;
     const int* recI;
     int teamWidth; // widest reaction side (the record holds kTeamWidth)
+    BlockTables blk;
 };
 
 constexpr int kSetsT = 16;

@@ -51,7 +51,7 @@ void put(std::vector<char>& blob, size_t& off, const T* src, size_t n) {
 // Uploads the mechanism tables with the reactions in the order `perm`
 // (perm[r] = reference index of the reaction stored at r).
 static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
-                         DeviceMech& d, void*& block) {
+                         DeviceMech& d, void*& block, bool blockTables = false) {
     const int nS = v.n_species, nR = v.n_reactions;
     std::vector<double> R(nS);
     for (int i = 0; i < nS; ++i) R[i] = kRR / v.W[i];
@@ -151,6 +151,87 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
             I(riRei + q) = okr ? rei[kWidth * r + q] : 0;
         }
     }
+    // gather lists of the workgroup-per-cell kernel
+    std::vector<int> spPtr(1, 0), spR, elPtr(1, 0), elRS, tbPtr(1, 0), tbR, tbSet;
+    std::vector<int> rbPtr(1, 0), rbEnt;
+    std::vector<double> spNu, elNu, tbNu, effB, effT, rbNu;
+    if (blockTables) {
+        // net coefficient of specie i in reaction r
+        auto nu = [&](int r, int i) {
+            double v = 0.0;
+            for (int q = 0; q < nl[r]; ++q)
+                if (li[kWidth * r + q] == i) v -= ls[kWidth * r + q];
+            for (int q = 0; q < nr[r]; ++q)
+                if (ri[kWidth * r + q] == i) v += rs[kWidth * r + q];
+            return v;
+        };
+        std::vector<std::vector<std::pair<int, double>>> rows(nS);
+        for (int r = 0; r < nR; ++r) {
+            std::vector<int> sp;
+            for (int q = 0; q < nl[r]; ++q) sp.push_back(li[kWidth * r + q]);
+            for (int q = 0; q < nr[r]; ++q) sp.push_back(ri[kWidth * r + q]);
+            std::sort(sp.begin(), sp.end());
+            sp.erase(std::unique(sp.begin(), sp.end()), sp.end());
+            for (int i : sp) {
+                const double c = nu(r, i);
+                if (c != 0.0) rows[i].push_back({r, c});
+            }
+        }
+        for (int i = 0; i < nS; ++i) {
+            for (auto& e : rows[i]) {
+                spR.push_back(e.first);
+                spNu.push_back(e.second);
+                if (type[e.first] == GFC_RATE_THIRD_BODY) {
+                    tbR.push_back(e.first);
+                    tbSet.push_back(effSet[e.first]);
+                    tbNu.push_back(e.second);
+                }
+            }
+            spPtr.push_back((int)spR.size());
+            tbPtr.push_back((int)tbR.size());
+        }
+        // entry lists: for each (i, j), the reactions of row i in which
+        // specie j is a reactant (slot q) or product (slot 3+q)
+        std::vector<std::vector<std::pair<int, double>>> el((size_t)nS * nS);
+        for (int i = 0; i < nS; ++i)
+            for (auto& e : rows[i]) {
+                const int r = e.first;
+                for (int q = 0; q < nl[r]; ++q)
+                    el[(size_t)i * nS + li[kWidth * r + q]].push_back({r * 8 + q, e.second});
+                for (int q = 0; q < nr[r]; ++q)
+                    el[(size_t)i * nS + ri[kWidth * r + q]].push_back({r * 8 + 3 + q, e.second});
+            }
+        elPtr.reserve((size_t)nS * nS + 1);
+        for (size_t k = 0; k < el.size(); ++k) {
+            for (auto& e : el[k]) {
+                elRS.push_back(e.first);
+                elNu.push_back(e.second);
+            }
+            elPtr.push_back((int)elRS.size());
+        }
+        // by row and 32-column block, ordered by column, then reaction
+        const int nCB = (nS + 31) / 32;
+        for (int i = 0; i < nS; ++i)
+            for (int b = 0; b < nCB; ++b) {
+                for (int jl = 0; jl < 32; ++jl) {
+                    const int j = 32 * b + jl;
+                    if (j >= nS) break;
+                    for (auto& e : el[(size_t)i * nS + j]) {
+                        rbEnt.push_back((jl << 24) | e.first);
+                        rbNu.push_back(e.second);
+                    }
+                }
+                rbPtr.push_back((int)rbEnt.size());
+            }
+        const size_t nSet = sets.size();
+        effB.assign(std::max<size_t>(1, nSet) * nS, 0.0);
+        effT.assign((size_t)nS * std::max<size_t>(1, nSet), 0.0);
+        for (size_t q = 0; q < nSet; ++q)
+            for (int i = 0; i < nS; ++i) {
+                effB[q * nS + i] = sets[q][i];
+                effT[(size_t)i * nSet + q] = sets[q][i];
+            }
+    }
     std::vector<char> blob;
     size_t off = 0;
     std::vector<size_t> offs;
@@ -192,6 +273,21 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     add(effSet.data(), effSet.size());
     add(recD.data(), recD.size());
     add(recI.data(), recI.size());
+    add(spPtr.data(), spPtr.size());
+    add(spR.data(), spR.size());
+    add(spNu.data(), spNu.size());
+    add(elPtr.data(), elPtr.size());
+    add(elRS.data(), elRS.size());
+    add(elNu.data(), elNu.size());
+    add(tbPtr.data(), tbPtr.size());
+    add(tbR.data(), tbR.size());
+    add(tbSet.data(), tbSet.size());
+    add(tbNu.data(), tbNu.size());
+    add(effB.data(), effB.size());
+    add(effT.data(), effT.size());
+    add(rbPtr.data(), rbPtr.size());
+    add(rbEnt.data(), rbEnt.size());
+    add(rbNu.data(), rbNu.size());
     if (hipFail(hipMalloc(&block, blob.size()), "hipMalloc(mechanism)"))
         return GFC_ERR_HIP;
     if (hipFail(hipMemcpy(block, blob.data(), blob.size(), hipMemcpyHostToDevice),
@@ -220,6 +316,11 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     at(d.effSet);
     at(d.recD);
     at(d.recI);
+    at(d.blk.spPtr); at(d.blk.spR); at(d.blk.spNu);
+    at(d.blk.elPtr); at(d.blk.elRS); at(d.blk.elNu);
+    at(d.blk.tbPtr); at(d.blk.tbR); at(d.blk.tbSet); at(d.blk.tbNu);
+    at(d.blk.effB); at(d.blk.effT);
+    at(d.blk.rbPtr); at(d.blk.rbEnt); at(d.blk.rbNu);
     d.teamWidth = teamWidth;
     d.nPass = nPass;
     d.nSets = static_cast<int>(sets.size());
@@ -247,6 +348,19 @@ SolveArgs solve_args(const gfc_evaluator* ev, double deltaT, double deltaTChemMa
     return A;
 }
 
+// Mechanisms the workgroup-per-cell kernel can run (its distinct
+// third-body sets are checked again at launch).
+bool block_eligible(const gfc_mech_view& v) {
+    const int n = v.n_species + 1;
+    if (n <= 64 || n > kBlockMaxN || v.n_reactions > kBlockMaxR) return false;
+    int width = 0, nonArr = 0;
+    for (int r = 0; r < v.n_reactions; ++r) {
+        width = std::max({width, v.n_lhs[r], v.n_rhs[r]});
+        nonArr += v.type[r] != GFC_RATE_ARRHENIUS;
+    }
+    return width <= kTeamWidth && nonArr <= kBlockMaxSets;
+}
+
 // Uploads both table orders from the evaluator's copy of the mechanism
 // (replacing earlier uploads).
 int evaluator_upload(gfc_evaluator* ev) {
@@ -268,7 +382,7 @@ int evaluator_upload(gfc_evaluator* ev) {
     };
     std::stable_sort(perm.begin(), perm.end(),
                      [&](int a, int b) { return rank(a) < rank(b); });
-    st = upload_tables(v, perm, ev->dmTeam, ev->mechBlockTeam);
+    st = upload_tables(v, perm, ev->dmTeam, ev->mechBlockTeam, block_eligible(v));
     if (st != GFC_OK) return st;
     return GFC_OK;
 }

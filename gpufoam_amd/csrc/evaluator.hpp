@@ -45,6 +45,7 @@ namespace gfc {
 SolveArgs solve_args(const gfc_evaluator* ev, double deltaT, double deltaTChemMax,
                      int nCells);
 int evaluator_upload(gfc_evaluator* ev);
+bool block_eligible(const gfc_mech_view& v);
 void adjoint_free(gfc_evaluator* ev);
 int batched_init(gfc_evaluator* ev);
 void batched_free(gfc_evaluator* ev);

@@ -86,6 +86,13 @@ struct SolveArgs {
     int* stats;                    // [3*ldY] or null
 };
 
+// Capacity of the workgroup-per-cell kernel (block_kernel.hip): n =
+// nSpecie+1 in (64, kBlockMaxN], at most kBlockMaxR reactions (net rates in
+// LDS) and kBlockMaxSets distinct third-body efficiency sets.
+constexpr int kBlockMaxN = 208;
+constexpr int kBlockMaxR = 1280;
+constexpr int kBlockMaxSets = 512;
+
 __device__ __forceinline__ void atomic_min_positive(unsigned long long* p,
                                                     double v) {
     atomicMin(p, static_cast<unsigned long long>(__double_as_longlong(v)));

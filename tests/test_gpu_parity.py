@@ -62,7 +62,8 @@ def _device_run(m, Y0, dt0, c, ode, rho, batched, graph=False, deltaTChemMax=1e3
             dRR.cpu().numpy())
 
 
-def _check(m, got, ref, tight_dt, Y0=None, rho=None, deltaT=None):
+def _check(m, got, ref, tight_dt, Y0=None, rho=None, deltaT=None, dt_rtol=1e-6,
+           min_same=None):
     Yg, dtg, dming, stg, rrg = got
     Yo, dto, dmino, sto, rro = ref
     nS = m.n_species
@@ -76,10 +77,10 @@ def _check(m, got, ref, tight_dt, Y0=None, rho=None, deltaT=None):
     same = stg[1] == sto[1]
     frac = same.mean()
     print(f"identical step sequence in {frac:.3f} of cells")
-    assert frac >= (1.0 if tight_dt else 0.8)
-    assert np.allclose(dtg[same], dto[same], rtol=1e-6, atol=0)
+    assert frac >= (min_same if min_same is not None else (1.0 if tight_dt else 0.8))
+    assert np.allclose(dtg[same], dto[same], rtol=dt_rtol, atol=0)
     if tight_dt:
-        assert np.isclose(dming, dmino, rtol=1e-6)
+        assert np.isclose(dming, dmino, rtol=dt_rtol)
     if Y0 is not None:
         # RR = rho (Ynew - Y0) / deltaT of the kernel's own Ynew
         # (gpuKernelEvaluator.cu:84-89), and within the state tolerance of
@@ -252,4 +253,8 @@ def test_synthetic_200_species_matches_oracle():
     for batched in (False, True):
         print(f"case syn200 n=4 batched={batched}", flush=True)
         got = _device_run(m, Y0, dt0, c, ode, rho, batched=batched)
-        _check(m, got, ref, tight_dt=True, Y0=Y0, rho=rho, deltaT=c["deltaT"])
+        # the batched form runs the workgroup-per-cell kernel, whose blocked
+        # summation order moves the error estimate (and the next chemical
+        # time step) at the 1e-5 level; see test_gpu_block.py
+        _check(m, got, ref, tight_dt=True, Y0=Y0, rho=rho, deltaT=c["deltaT"],
+               dt_rtol=1e-4 if batched else 1e-6)

@@ -106,10 +106,9 @@ int batched_init(gfc_evaluator* ev) {
                     "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
             return GFC_ERR_HIP;
         if (occ > 0) {
-            const long nRp = ((long)ev->nR + 63) / 64 * 64;
             b->blkNA = NA;
             b->blkGrid = b->numCUs * occ;
-            b->blkPerWG = 8 * nRp + 2L * NA * NC * kBlockThreads;
+            b->blkPerWG = 2L * NA * NC * kBlockThreads;
             if (hipFail(hipMalloc(&b->blkScratch, sizeof(double) * b->blkPerWG * b->blkGrid),
                         "hipMalloc(workgroup scratch)"))
                 return GFC_ERR_HIP;

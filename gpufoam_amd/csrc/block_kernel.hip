@@ -15,11 +15,10 @@
 //     net rates go to LDS and dN/dt is a specie-parallel gather over each
 //     specie's reactions (no atomics, so the sums are deterministic);
 //   * Jacobian (gpuODESystem.H:198-281 with gpuReaction.H:547-612): the
-//     per-reaction concentration derivatives go to a per-workgroup global
-//     cache; each thread gathers its own entries from host-built lists
-//     (sparse part per entry, dense third-body part per row), then applies
-//     the Y / T transforms; the result is kept in global scratch and
-//     reloaded by every attempt;
+//     per-reaction concentration derivatives go to LDS; each thread gathers
+//     its own entries from host-built lists (sparse part per entry, dense
+//     third-body part per row), then applies the Y / T transforms; the
+//     result is kept in global scratch and reloaded by every attempt;
 //   * LU (ludecompose.H:9-92): right-looking elimination with the
 //     reference's implicit row scaling and pivot rule (largest scaled value,
 //     ties to the last row position, zero pivot -> DBL_EPSILON); rows are
@@ -43,9 +42,6 @@ constexpr int kBT = 512;          // threads per workgroup
 constexpr int kBW = kBT / 64;     // wavefronts per workgroup
 constexpr int kPR = 16;           // thread rows
 constexpr int kPC = 32;           // thread columns
-constexpr int kCacheF = 8;        // cached per-reaction fields
-
-enum { cfD0 = 0, cfG = 6, cfDwdT = 7 };
 
 template <int NA, int NC>
 struct BlockLDS {
@@ -58,7 +54,9 @@ struct BlockLDS {
     double k1[NV], k2[NV], k3[NV], k4[NV];
     double c[NV], Ha[NV], Cp[NV], colA[NV], colB[NV], JT[NV], aux[NV], rowT[NV], vv[NV];
     double gL[NV], gH[NV], hL[NV], hH[NV];
-    double w[kBlockMaxR];
+    double w[kBlockMaxR];    // net rate of each reaction
+    double dwdT[kBlockMaxR]; // its temperature derivative (Jacobian only)
+    double G[kBlockMaxR];    // dense third-body weight k (Cf - Cr/Kc)
     double M[kBlockMaxSets];
     double colbuf[2][NV];
     double rdiag[NV];
@@ -267,99 +265,13 @@ __device__ __forceinline__ void blk_derivatives(const DeviceMech& m, double p, c
 }
 
 // ------------------------------------------------------------- jacobian --
-// One 32-column block of the concentration Jacobian (gpuReaction.H:547-612
-// with the dense third-body part): each specie row is assembled in LDS by one
-// thread (third-body part, then the sparse contributions of each column in
-// reaction order), then every thread transforms its own entries of the
-// block (ddY_dcdY, gpuODESystem.H:65-103), adds them to the temperature-row
-// column sums (ddT_dtdY, :108-126) and stores them.
-template <int NA, int NC, int B>
-__device__ __forceinline__ void jac_block(const DeviceMech& m, BlockLDS<NA, NC>& L,
-                                          const double* cache, long nRp, double* Jw,
-                                          double rhoM, double dfdxT, double CpM,
-                                          double (&rp)[NA], const Coord& q) {
-    const int nS = m.nS, t = q.tid;
-    double* Jb = L.R; // [row][33]
-    if (kPC * B < nS && t < nS) {
-        const int nCB = (nS + 31) >> 5;
-        double dacc[32];
-#pragma unroll
-        for (int jl = 0; jl < 32; ++jl) dacc[jl] = 0.0;
-        const int te = m.blk.tbPtr[t + 1];
-        for (int e = m.blk.tbPtr[t]; e < te; ++e) {
-            const double g = m.blk.tbNu[e] * cache[cfG * nRp + m.blk.tbR[e]];
-            const double* ef = m.blk.effB + (long)m.blk.tbSet[e] * nS + kPC * B;
-#pragma unroll
-            for (int jl = 0; jl < 32; ++jl)
-                if (kPC * B + jl < nS) dacc[jl] += g * ef[jl];
-        }
-        double* row = Jb + t * 33;
-#pragma unroll
-        for (int jl = 0; jl < 32; ++jl) row[jl] = dacc[jl];
-        const int se = m.blk.rbPtr[t * nCB + B + 1];
-        int cur = -1;
-        double acc = 0.0;
-#pragma unroll 4
-        for (int e = m.blk.rbPtr[t * nCB + B]; e < se; ++e) {
-            const int en = m.blk.rbEnt[e];
-            const int jl = en >> 24, rs = en & 0xffffff;
-            const double v = m.blk.rbNu[e] * cache[(rs & 7) * nRp + (rs >> 3)];
-            if (jl != cur) {
-                if (cur >= 0) row[cur] += acc;
-                cur = jl;
-                acc = 0.0;
-            }
-            acc += v;
-        }
-        if (cur >= 0) row[cur] += acc;
-    }
-    __syncthreads();
-    const int j = q.tc + kPC * B;
-    double v = 0.0;
-#pragma unroll
-    for (int a = 0; a < NA; ++a) {
-        const int i = q.tr + kPR * a;
-        double out = 0.0;
-        if (i < nS && j < nS) {
-            const double tj = L.poison[i] ? __longlong_as_double(0x7ff8000000000000LL)
-                                          : Jb[i * 33 + q.tc];
-            rp[a] += tj * L.c[j];
-            out = (m.W[i] / rhoM) * tj * L.colA[j] + L.colB[j] * L.dfdx[i];
-            v += out * L.Ha[i];
-        }
-        Jw[(a * NC + B) * 64 + q.lane] = out;
-    }
-    // temperature row of this column: sum over the 16 thread-rows
-    v += dpp<0x111>(0.0, v);
-    v += dpp<0x112>(0.0, v);
-    v += dpp<0x114>(0.0, v);
-    v += dpp<0x118>(0.0, v);
-    v = __shfl(v, (q.lane & ~15) | 15);
-    if (j < nS && q.tr == (nS & 15)) {
-#pragma unroll
-        for (int a = 0; a < NA; ++a)
-            if (q.tr + kPR * a == nS) Jw[(a * NC + B) * 64 + q.lane] = (-v - L.Cp[j] * dfdxT) / CpM;
-    }
-    __syncthreads();
-}
-
-template <int NA, int NC, int B = 0>
-__device__ __forceinline__ void jac_all(const DeviceMech& m, BlockLDS<NA, NC>& L,
-                                        const double* cache, long nRp, double* Jw, double rhoM,
-                                        double dfdxT, double CpM, double (&rp)[NA],
-                                        const Coord& q) {
-    if constexpr (B < NC) {
-        jac_block<NA, NC, B>(m, L, cache, nRp, Jw, rhoM, dfdxT, CpM, rp, q);
-        jac_all<NA, NC, B + 1>(m, L, cache, nRp, Jw, rhoM, dfdxT, CpM, rp, q);
-    }
-}
-
 // gpuODESystem::jacobian (gpuODESystem.H:198-281): leaves J (n x n) in the
 // per-thread layout of Jg and its derivative (the reference's dfdx) in
-// L.dfdx.  cache: per-workgroup global [kCacheF][nRp].
+// L.dfdx.  The per-reaction derivatives D[q][r] (q < 6 specie positions)
+// occupy L.R during the assembly.
 template <int NA, int NC>
 __device__ __forceinline__ void blk_jacobian(const DeviceMech& m, double p, BlockLDS<NA, NC>& L,
-                             double* cache, long nRp, double* Jg, int& tog, const Coord& q0) {
+                             long nRp, double* Jg, int& tog, const Coord& q0) {
     const Coord q = opaque(q0);
     constexpr int NV = BlockLDS<NA, NC>::NV;
     const int nS = m.nS, t = q.tid;
@@ -398,6 +310,7 @@ __device__ __forceinline__ void blk_jacobian(const DeviceMech& m, double p, Bloc
     third_body_sets(m, L, q);
     __syncthreads();
     // reaction-parallel: rates, dN/dt terms and concentration derivatives
+    double* D = L.R;
     for (int r = t; r < m.nR; r += kBT) {
         const Rxn x = load_rxn(m, r >> 6, r & 63);
         double kArr, ddT, pwl[kTeamWidth], pwr[kTeamWidth];
@@ -419,13 +332,15 @@ __device__ __forceinline__ void blk_jacobian(const DeviceMech& m, double p, Bloc
         }
         const double temp = (Cf - Cr / Kc);
         L.w[r] = w;
-        cache[cfDwdT * nRp + r] = dwdT;
-        cache[cfG * nRp + r] = (x.type == GFC_RATE_THIRD_BODY) ? kArr * temp : 0.0;
+        L.dwdT[r] = dwdT;
+        L.G[r] = (x.type == GFC_RATE_THIRD_BODY) ? kArr * temp : 0.0;
+        // dw/dc of the specie at each position (left side kf dCf/dc, right
+        // side -kr dCr/dc) in D[q][r] = L.R[q*nRp + r]
 #pragma unroll
         for (int j = 0; j < kTeamWidth; ++j) {
-            cache[(cfD0 + j) * nRp + r] =
+            D[j * nRp + r] =
                 (j < x.nl) ? side_dpow(x.li, x.lei, x.le, x.nl, j, L.c, pwl) * kf : 0.0;
-            cache[(cfD0 + kTeamWidth + j) * nRp + r] =
+            D[(kTeamWidth + j) * nRp + r] =
                 (!x.irr && j < x.nr) ? -(side_dpow(x.ri, x.rei, x.re, x.nr, j, L.c, pwr) * kr)
                                      : 0.0;
         }
@@ -451,7 +366,7 @@ __device__ __forceinline__ void blk_jacobian(const DeviceMech& m, double p, Bloc
             const int r = m.blk.spR[e];
             const double nu = m.blk.spNu[e];
             dN += nu * L.w[r];
-            JT += nu * cache[cfDwdT * nRp + r];
+            JT += nu * L.dwdT[r];
         }
         d = dN * (m.W[t] / rhoM);
         L.dfdx[t] = d;
@@ -461,14 +376,86 @@ __device__ __forceinline__ void blk_jacobian(const DeviceMech& m, double p, Bloc
     block_sum(sT, L, tog, q);
     const double dfdxT = -sT[0] / CpM;
     if (t == nS) L.dfdx[nS] = dfdxT;
-    // concentration Jacobian by 32-column blocks, transformed and stored
-    // block by block (only the row sums of the temperature column carry
-    // over)
-    double rp[NA];
-#pragma unroll
-    for (int a = 0; a < NA; ++a) rp[a] = 0.0;
+    // this thread's entries: the concentration derivatives gathered from
+    // the per-reaction values in LDS (sparse part in reaction order, then
+    // the dense third-body part), transformed (ddY_dcdY,
+    // gpuODESystem.H:65-103) and stored; the row sums of the temperature
+    // column and the column sums of the temperature row (ddT_dtdY, :108-126)
+    // are accumulated on the way
     double* Jw = wave_slots<NA * NC>(Jg, q);
-    jac_all(m, L, cache, nRp, Jw, rhoM, dfdxT, CpM, rp, q);
+    double rp[NA], cs[NC];
+#pragma unroll
+    for (int b = 0; b < NC; ++b) cs[b] = 0.0;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        const int i = q.tr + kPR * a;
+        rp[a] = 0.0;
+        double v[NC];
+#pragma unroll
+        for (int b = 0; b < NC; ++b) v[b] = 0.0;
+        if (i < nS) {
+            int e0[NC], e1[NC];
+#pragma unroll
+            for (int b = 0; b < NC; ++b) {
+                const int j = q.tc + kPC * b;
+                const int* ep = m.blk.elPtr + (long)i * nS + (j < nS ? j : 0);
+                e0[b] = ep[0];
+                e1[b] = (j < nS) ? ep[1] : e0[b];
+            }
+#pragma unroll
+            for (int b = 0; b < NC; ++b) {
+#pragma unroll 2
+                for (int e = e0[b]; e < e1[b]; ++e) {
+                    const int rs = m.blk.elRS[e];
+                    v[b] += m.blk.elNu[e] * D[(rs & 7) * nRp + (rs >> 3)];
+                }
+            }
+            const int te = m.blk.tbPtr[i + 1];
+            for (int e = m.blk.tbPtr[i]; e < te; ++e) {
+                const double g = m.blk.tbNu[e] * L.G[m.blk.tbR[e]];
+                const double* ef = m.blk.effB + (long)m.blk.tbSet[e] * nS;
+#pragma unroll
+                for (int b = 0; b < NC; ++b) {
+                    const int j = q.tc + kPC * b;
+                    if (j < nS) v[b] += g * ef[j];
+                }
+            }
+            const bool bad = L.poison[i] != 0;
+            const double WiByrhoM = m.W[i] / rhoM;
+            const double di = L.dfdx[i];
+            const double Hai = L.Ha[i];
+#pragma unroll
+            for (int b = 0; b < NC; ++b) {
+                const int j = q.tc + kPC * b;
+                if (j < nS) {
+                    const double tj = bad ? __longlong_as_double(0x7ff8000000000000LL) : v[b];
+                    rp[a] += tj * L.c[j];
+                    v[b] = WiByrhoM * tj * L.colA[j] + L.colB[j] * di;
+                    cs[b] += v[b] * Hai;
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < NC; ++b) Jw[(a * NC + b) * 64 + q.lane] = v[b];
+    }
+    // temperature row: column sums over the 16 thread-rows of the column
+#pragma unroll
+    for (int b = 0; b < NC; ++b) {
+        const int j = q.tc + kPC * b;
+        double v = cs[b];
+        v += dpp<0x111>(0.0, v);
+        v += dpp<0x112>(0.0, v);
+        v += dpp<0x114>(0.0, v);
+        v += dpp<0x118>(0.0, v);
+        v = __shfl(v, (q.lane & ~15) | 15);
+        if (j < nS && q.tr == (nS & 15)) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+                if (q.tr + kPR * a == nS)
+                    Jw[(a * NC + b) * 64 + q.lane] = (-v - L.Cp[j] * dfdxT) / CpM;
+        }
+    }
+    __syncthreads();
     // ddNdT: the row sums reduced through LDS in thread-column order
 #pragma unroll
     for (int a = 0; a < NA; ++a) L.R[q.tc * NV + q.tr + kPR * a] = rp[a];
@@ -856,8 +843,7 @@ block_solve_kernel(SolveArgs A, int* queue, double* scratch, long perWG) {
     const long ld = A.ldY;
     const RosTableau& tb = A.tab;
     const long nRp = ((long)m.nR + 63) / 64 * 64;
-    double* cache = scratch + (long)blockIdx.x * perWG;
-    double* Jg = cache + kCacheF * nRp;
+    double* Jg = scratch + (long)blockIdx.x * perWG;
     double* LUg = Jg + (long)NA * NC * kBT;
     int tog = 0;
 
@@ -888,7 +874,7 @@ block_solve_kernel(SolveArgs A, int* queue, double* scratch, long perWG) {
                     // adaptiveSolve (gpuAdaptiveSolver.H:83-134)
                     double dx = stepDx;
                     const double T = L.y[nS];
-                    blk_jacobian(m, p, L, cache, nRp, Jg, tog, q);
+                    blk_jacobian(m, p, L, nRp, Jg, tog, q);
                     __syncthreads();
                     // derivative at y0: the Jacobian's own unless T is outside
                     // the reaction temperature limits (omega() clips it there)
@@ -1019,6 +1005,8 @@ block_solve_kernel(SolveArgs A, int* queue, double* scratch, long perWG) {
     }
 }
 
+static_assert(BlockLDS<8, 4>::RSZ == block_lds_doubles(128), "LDS region of n <= 128");
+static_assert(BlockLDS<13, 7>::RSZ == block_lds_doubles(208), "LDS region of n <= 208");
 template __global__ void block_solve_kernel<8, 4>(SolveArgs, int*, double*, long);
 template __global__ void block_solve_kernel<13, 7>(SolveArgs, int*, double*, long);
 

@@ -28,12 +28,6 @@ struct BlockTables {
     const int* elPtr;
     const int* elRS;
     const double* elNu;
-    // the same contributions by row and 32-column block, sorted by column
-    // then reaction: rbPtr[i*nCB + b] .. rbPtr[i*nCB + b + 1] (nCB = column
-    // blocks of the specie columns), rbEnt = (column in block << 24) | (r*8 + q)
-    const int* rbPtr;
-    const int* rbEnt;
-    const double* rbNu;
     // third-body reactions of each specie row: J(i,j) += tbNu g_r effB[set][j]
     const int* tbPtr;
     const int* tbR;

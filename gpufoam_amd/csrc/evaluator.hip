@@ -153,8 +153,7 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     }
     // gather lists of the workgroup-per-cell kernel
     std::vector<int> spPtr(1, 0), spR, elPtr(1, 0), elRS, tbPtr(1, 0), tbR, tbSet;
-    std::vector<int> rbPtr(1, 0), rbEnt;
-    std::vector<double> spNu, elNu, tbNu, effB, effT, rbNu;
+    std::vector<double> spNu, elNu, tbNu, effB, effT;
     if (blockTables) {
         // net coefficient of specie i in reaction r
         auto nu = [&](int r, int i) {
@@ -209,20 +208,6 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
             }
             elPtr.push_back((int)elRS.size());
         }
-        // by row and 32-column block, ordered by column, then reaction
-        const int nCB = (nS + 31) / 32;
-        for (int i = 0; i < nS; ++i)
-            for (int b = 0; b < nCB; ++b) {
-                for (int jl = 0; jl < 32; ++jl) {
-                    const int j = 32 * b + jl;
-                    if (j >= nS) break;
-                    for (auto& e : el[(size_t)i * nS + j]) {
-                        rbEnt.push_back((jl << 24) | e.first);
-                        rbNu.push_back(e.second);
-                    }
-                }
-                rbPtr.push_back((int)rbEnt.size());
-            }
         const size_t nSet = sets.size();
         effB.assign(std::max<size_t>(1, nSet) * nS, 0.0);
         effT.assign((size_t)nS * std::max<size_t>(1, nSet), 0.0);
@@ -285,9 +270,6 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     add(tbNu.data(), tbNu.size());
     add(effB.data(), effB.size());
     add(effT.data(), effT.size());
-    add(rbPtr.data(), rbPtr.size());
-    add(rbEnt.data(), rbEnt.size());
-    add(rbNu.data(), rbNu.size());
     if (hipFail(hipMalloc(&block, blob.size()), "hipMalloc(mechanism)"))
         return GFC_ERR_HIP;
     if (hipFail(hipMemcpy(block, blob.data(), blob.size(), hipMemcpyHostToDevice),
@@ -320,7 +302,6 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     at(d.blk.elPtr); at(d.blk.elRS); at(d.blk.elNu);
     at(d.blk.tbPtr); at(d.blk.tbR); at(d.blk.tbSet); at(d.blk.tbNu);
     at(d.blk.effB); at(d.blk.effT);
-    at(d.blk.rbPtr); at(d.blk.rbEnt); at(d.blk.rbNu);
     d.teamWidth = teamWidth;
     d.nPass = nPass;
     d.nSets = static_cast<int>(sets.size());
@@ -353,6 +334,10 @@ SolveArgs solve_args(const gfc_evaluator* ev, double deltaT, double deltaTChemMa
 bool block_eligible(const gfc_mech_view& v) {
     const int n = v.n_species + 1;
     if (n <= 64 || n > kBlockMaxN || v.n_reactions > kBlockMaxR) return false;
+    // the per-reaction derivatives (6 per reaction) share LDS with the
+    // diagonal factor blocks during the Jacobian assembly
+    const long nRp = ((long)v.n_reactions + 63) / 64 * 64;
+    if (6 * nRp > block_lds_doubles(n)) return false;
     int width = 0, nonArr = 0;
     for (int r = 0; r < v.n_reactions; ++r) {
         width = std::max({width, v.n_lhs[r], v.n_rhs[r]});

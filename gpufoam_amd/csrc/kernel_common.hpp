@@ -93,6 +93,15 @@ constexpr int kBlockMaxN = 208;
 constexpr int kBlockMaxR = 1280;
 constexpr int kBlockMaxSets = 512;
 
+// Doubles of the kernel's shared LDS region (diagonal factor blocks / row
+// sums / per-reaction derivatives) for n = nSpecie+1: 16 x NA rows and
+// NC 32-wide column blocks with NA = 8, NC = 4 up to n = 128 and
+// NA = 13, NC = 7 above.
+constexpr long block_lds_doubles(int n) {
+    return n <= 128 ? (4 * 32 * 33 > 32 * 128 ? 4 * 32 * 33 : 32 * 128)
+                    : (7 * 32 * 33 > 32 * 208 ? 7 * 32 * 33 : 32 * 208);
+}
+
 __device__ __forceinline__ void atomic_min_positive(unsigned long long* p,
                                                     double v) {
     atomicMin(p, static_cast<unsigned long long>(__double_as_longlong(v)));

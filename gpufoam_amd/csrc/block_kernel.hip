@@ -504,32 +504,61 @@ __device__ __forceinline__ void lu_cols(double (&A)[NA][NC], int n, BlockLDS<NA,
         if (q.tc == kk) {
             // pivot search over column k: (scaled value, position),
             // larger wins; rows already pivots and padding never win,
-            // NaN keys lose to every number.  The winner's row and value
-            // travel with it through the reduction.
+            // NaN keys lose to every number.
             double vvv[NA];
-            int psv[NA];
 #pragma unroll
-            for (int a = 0; a < NA; ++a) {
-                vvv[a] = L.vv[q.tr + kPR * a];
-                psv[a] = L.pos[q.tr + kPR * a];
-            }
-            double bkey = -2.0, bval = 0.0;
-            int bpos = -1, brow = 0;
+            for (int a = 0; a < NA; ++a) vvv[a] = L.vv[q.tr + kPR * a];
+            // keys and their maximum over the 16 lanes (row_ror)
+            double keyv[NA];
+            double kmax = -2.0;
 #pragma unroll
             for (int a = 0; a < NA; ++a) {
                 const int i = q.tr + kPR * a;
                 const bool cand = i < n && !((pivoted >> a) & 1u);
                 double key = vvv[a] * fabs(A[a][bK]);
                 if (!(key >= 0.0)) key = -1.0;
-                if (!cand) key = -2.0;
-                const int ps = cand ? psv[a] : -1;
-                const bool better = key > bkey || (key == bkey && ps > bpos);
-                bkey = better ? key : bkey;
-                bpos = better ? ps : bpos;
-                brow = better ? i : brow;
-                bval = better ? A[a][bK] : bval;
+                keyv[a] = cand ? key : -2.0;
+                kmax = fmax(kmax, keyv[a]);
             }
-            // argmax over the 16 lanes of the thread-column (row_ror)
+            kmax = fmax(kmax, dpp<0x121>(kmax, kmax));
+            kmax = fmax(kmax, dpp<0x122>(kmax, kmax));
+            kmax = fmax(kmax, dpp<0x124>(kmax, kmax));
+            kmax = fmax(kmax, dpp<0x128>(kmax, kmax));
+            // (lane, row) pairs holding the maximum: the usual single one
+            // is the pivot; several (a tie, or only NaN/zero keys) go to the
+            // position rule below
+            int nm = 0, aStar = 0;
+            unsigned long long mStar = 0ull;
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                const unsigned long long mk = __ballot(keyv[a] == kmax);
+                nm += __builtin_popcountll(mk);
+                if (mk) {
+                    aStar = a;
+                    mStar = mk;
+                }
+            }
+            int Pw, p;
+            double dj;
+            if (nm == 1) {
+                const int lStar = __builtin_ctzll(mStar);
+                p = (lStar & 15) + kPR * aStar;
+                dj = readlane(row_pick<NA, NC, bK>(A, aStar), lStar);
+                Pw = -1; // read from the position map below
+            } else {
+                // largest (key, position); the row and value travel along
+                double bkey = -2.0, bval = 0.0;
+                int bpos = -1, brow = 0;
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                    const int i = q.tr + kPR * a;
+                    const int ps = (keyv[a] > -2.0) ? L.pos[i] : -1;
+                    const bool better = keyv[a] > bkey || (keyv[a] == bkey && ps > bpos);
+                    bkey = better ? keyv[a] : bkey;
+                    bpos = better ? ps : bpos;
+                    brow = better ? i : brow;
+                    bval = better ? A[a][bK] : bval;
+                }
 #define GFC_ROR_STEP(CTRL)                                                   \
     {                                                                        \
         const double ok = dpp<CTRL>(bkey, bkey);                             \
@@ -542,15 +571,16 @@ __device__ __forceinline__ void lu_cols(double (&A)[NA][NC], int n, BlockLDS<NA,
         brow = better ? orw : brow;                                          \
         bval = better ? ov : bval;                                           \
     }
-            GFC_ROR_STEP(0x121)
-            GFC_ROR_STEP(0x122)
-            GFC_ROR_STEP(0x124)
-            GFC_ROR_STEP(0x128)
+                GFC_ROR_STEP(0x121)
+                GFC_ROR_STEP(0x122)
+                GFC_ROR_STEP(0x124)
+                GFC_ROR_STEP(0x128)
 #undef GFC_ROR_STEP
-            const int Pw = __builtin_amdgcn_readfirstlane(bpos);
-            const int p = __builtin_amdgcn_readfirstlane(brow);
+                Pw = __builtin_amdgcn_readfirstlane(bpos);
+                p = __builtin_amdgcn_readfirstlane(brow);
+                dj = readlane(bval, __builtin_amdgcn_readfirstlane(q.lane));
+            }
             const int aP = p >> 4;
-            double dj = readlane(bval, __builtin_amdgcn_readfirstlane(q.lane));
             if (dj == 0.0) {
                 dj = kSmall;
                 const bool me = q.tr == (p & 15);
@@ -569,6 +599,7 @@ __device__ __forceinline__ void lu_cols(double (&A)[NA][NC], int n, BlockLDS<NA,
                 L.colbuf[par][q.tr * NA + a] = l;
             }
             if (q.tr == 0) {
+                if (Pw < 0) Pw = L.pos[p];
                 const int qrow = L.perm[k];
                 L.perm[Pw] = qrow;
                 L.pos[qrow] = Pw;

@@ -131,7 +131,7 @@ GFC_HD inline double omega(const DeviceMech& m, int r, double T, SV c) {
 
 // gpuODESystem::derivatives (gpuODESystem.H:164-196) with
 // gpuMixture::computeDerivativeRequired (gpuMixture.H:88-121).
-GFC_HD inline __noinline__ void derivatives(const DeviceMech& m, double p, SV y, SV dydx,
+GFC_HD __forceinline__ void derivatives(const DeviceMech& m, double p, SV y, SV dydx,
                             LaneBuf b) {
     const int nS = m.nS;
     for (int i = 0; i <= nS; ++i) dydx[i] = 0;
@@ -163,7 +163,7 @@ GFC_HD inline __noinline__ void derivatives(const DeviceMech& m, double p, SV y,
 // gpuODESystem::jacobian (gpuODESystem.H:198-281) with
 // computeReactionParameters / ddNdtByVdcTp (gpuReaction.H:217-233,547-702)
 // and the Y/T transforms (gpuODESystem.H:65-146).
-GFC_HD inline __noinline__ void jacobian(const DeviceMech& m, double p, SV y, LaneBuf b) {
+GFC_HD __forceinline__ void jacobian(const DeviceMech& m, double p, SV y, LaneBuf b) {
     const int nS = m.nS, n = nS + 1;
     SV dfdx = b.dfdx();
     for (int i = 0; i < n; ++i) dfdx[i] = 0.0;
@@ -342,7 +342,7 @@ GFC_HD inline __noinline__ void jacobian(const DeviceMech& m, double p, SV y, La
 // ---------------------------------------------------------------- LU --
 
 // ludecompose.H:9-92 (Crout, implicit scaling, last maximal row wins)
-GFC_HD inline __noinline__ void lu_decompose(LaneBuf b) {
+GFC_HD __forceinline__ void lu_decompose(LaneBuf b) {
     const int m = b.n;
     for (int i = 0; i < m; i++) {
         double largest = 0.0;
@@ -388,7 +388,7 @@ GFC_HD inline __noinline__ void lu_decompose(LaneBuf b) {
 }
 
 // ludecompose.H:94-130
-GFC_HD inline __noinline__ void lu_backsub(LaneBuf b, SV x) {
+GFC_HD __forceinline__ void lu_backsub(LaneBuf b, SV x) {
     const int m = b.n;
     int ii = 0;
     for (int i = 0; i < m; i++) {

@@ -453,8 +453,11 @@ __device__ __forceinline__ double team_solve(const double (&row)[N], int lane, i
 
 // ---------------------------------------------------------------- kernel --
 
+// Small mechanisms (N <= 16) fit in 256 registers: two wavefronts per SIMD
+// hide each other's latency.  Larger ones need the AGPRs as well and run
+// one wavefront per SIMD.
 template <int N>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, (N <= 16) ? 2 : 1)
 team_solve_kernel(SolveArgs A, int* queue) {
     __shared__ TeamLDS<N> L;
     const int lane = threadIdx.x;

@@ -134,7 +134,7 @@ def main():
     if args.gpus != 1:
         print("bench.py measures one GPU only: the chemistry of each cell is "
               "independent, so N GPUs would each run this same kernel on their "
-              "share of cells (see DESIGN.md); no multi-GPU figure is produced.",
+              "share of cells; no multi-GPU figure is produced.",
               file=sys.stderr)
         sys.exit(2)
 

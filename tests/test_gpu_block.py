@@ -124,3 +124,32 @@ def test_block_kernel_deterministic_and_graph_replay():
     for r in runs[1:]:
         for a, b in zip(runs[0], r):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("solver", ["Rosenbrock12", "Rosenbrock23"])
+def test_block_kernel_other_solvers(solver):
+    """The two- and three-stage Rosenbrock tableaux through the
+    workgroup-per-cell kernel (100-species mechanism, relTol 1e-5)."""
+    m = _syn100()
+    Y0, dt0 = syn_states(m, 8, seed=12)
+    c = dict(solver=solver, relTol=1e-5, absTol=1e-10, deltaT=1e-6)
+    rho = np.linspace(0.5, 1.1, Y0.shape[1])
+    ode, ref = _oracle(m, Y0, dt0, c, rho)
+    print(f"case syn100 n=8 {solver}", flush=True)
+    got = _device_run(m, Y0, dt0, c, ode, rho, batched=True)
+    _check_sensitive(m, got, ref, Y0, rho, c["deltaT"])
+
+
+def test_block_kernel_reports_max_steps():
+    """Running out of maxSteps sets the status bit, as in the oracle."""
+    from gpufoam_amd.evaluator import Evaluator
+    from oracle_lib import Oracle
+    m = _syn100()
+    Y0, dt0 = syn_states(m, 8, seed=13)
+    ode = ode_inputs("Rosenbrock34", relTol=1e-6, absTol=1e-10, maxSteps=3)
+    ref = Oracle(m).solve_cells(ode, Y0, 1e-6, dt0)
+    print("case syn100 n=8 maxSteps=3", flush=True)
+    ev = Evaluator(m, ode, 0.0, max_cells=8)
+    _, _, _, st, _ = ev.solve_host(1e-6, Y0, dt0, batched=True)
+    assert np.array_equal(st[0], ref[3][0])
+    assert (st[0] & 1).any()

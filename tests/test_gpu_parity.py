@@ -241,8 +241,9 @@ def test_max_steps_is_reported():
 
 def test_synthetic_200_species_matches_oracle():
     """The 200-species / 1200-reaction synthetic mechanism (n = 201, beyond
-    the wavefront kernel's capacity: both forms run one lane per cell), on
-    four cells of its state law with short step sequences."""
+    the wavefront kernel's capacity: the plain form runs one lane per cell,
+    the batched form one workgroup per cell), on four cells of its state
+    law with short step sequences."""
     from oracle_lib import Oracle
     m, Y0, dt0, c = make_states("syn200", 16)
     keep = [1, 5, 8, 11]

@@ -127,6 +127,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--adjoint-cells", type=int, default=65536,
                     help="cells of the differentiation workload for the adjoint cost (0: skip)")
+    ap.add_argument("--autotune", type=int, default=0, metavar="CELLS",
+                    help="run the launch-shape autotuner on CELLS cells of the workload "
+                         "first and time the batched solve with the shape it picks")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times "
                          "(the known-size kernel that scales the HBM byte counters)")
@@ -166,6 +169,15 @@ def main():
     stats = torch.zeros((3, n), dtype=torch.int32, device=dev)
     ev = Evaluator(m, ode, 0.0, max_cells=n)
     graph = not args.no_graph
+    tuned = None
+    if args.autotune:
+        from gpufoam_amd.autotune import autotune
+        progress(f"autotuning on {args.autotune} cells")
+        times = autotune(ev, c["deltaT"], dY0, ddt0, sample=args.autotune, repeats=2)
+        best = times.pop("best")
+        tuned = {"sample_cells": args.autotune, "best": list(best),
+                 "ms": {f"{k[0]}x{k[1]}": round(v, 3) for k, v in times.items()}}
+        progress(f"autotuner: {tuned}")
 
     def step(with_stats=False):
         ddt.copy_(ddt0)
@@ -257,6 +269,7 @@ def main():
         "plain_sample_cells": ns,
         "batched_over_plain": value / plain_value if plain_value else None,
         "adjoint": adjoint,
+        "autotune": tuned,
         "attempts_per_cell": float(st[1].mean()),
         "accepted_per_cell": float(st[2].mean()),
         "wall_s": wall,

@@ -2,7 +2,8 @@
 
 The search space is the evaluator's batched shapes (gfc_evaluator_set_batched_shape):
 one wavefront per cell with 1, 2, 4, ... resident wavefronts per compute
-unit up to what fits, and one lane per cell.  Each shape is timed with HIP
+unit up to what fits, four cells per wavefront (16-lane teams, for
+mechanisms with nSpecie+1 <= 16), and one lane per cell.  Each shape is timed with HIP
 events on one CFD step of a sample of the caller's own cells (the chemical
 time step reset before every run, so every shape does the same work), and
 the fastest is set on the evaluator.
@@ -30,10 +31,14 @@ def autotune(ev, deltaT: float, Y0, dtChem, sample: int = 4096, repeats: int = 2
         Ys, dts = Y0[:, :n].contiguous(), dtChem.clone()
     Yn = torch.empty_like(Ys)
     if shapes is None:
-        shapes = [(1, w) for w in (1, 2, 4, 8, 16)] + [(1, 0), (64, 0)]
+        shapes = [(1, w) for w in (1, 2, 4, 8, 16)] + [(1, 0), (4, 0), (64, 0)]
+    from ._capi import GfcError
     times = {}
     for cpw, wpc in shapes:
-        ev.set_batched_shape(cpw, wpc)
+        try:
+            ev.set_batched_shape(cpw, wpc)
+        except GfcError:
+            continue  # a shape this mechanism cannot run (4 cells need n <= 16)
         d = dts.clone()
         ev.solve(deltaT, Ys, d, Ynew=Yn, batched=True)  # warm-up (and graph-free)
         ms = []

@@ -23,6 +23,8 @@ namespace gfc {
 
 template <int N>
 __global__ void team_solve_kernel(SolveArgs A, int* queue);
+template <int N>
+__global__ void team16_solve_kernel(SolveArgs A, int* queue);
 template <int NA, int NC>
 __global__ void block_solve_kernel(SolveArgs A, int* queue, double* scratch, long perWG);
 __global__ void lane_solve_kernel(SolveArgs A, double* scratch, long per);
@@ -76,6 +78,17 @@ KernelFn team_kernel_for(int n, int& N) {
     if (n <= 32) { N = 32; return team_solve_kernel<32>; }
     if (n <= 64) { N = 64; return team_solve_kernel<64>; }
     N = 0;
+    return nullptr;
+}
+
+// Four cells per wavefront (16-lane teams): n <= 16, at most kSetsT
+// efficiency sets, at most kTeamWidth species per reaction side.
+KernelFn team16_kernel_for(const DeviceMech& m, int& N) {
+    const int n = m.nS + 1;
+    N = 0;
+    if (m.nSets > kSetsT || m.teamWidth > kTeamWidth) return nullptr;
+    if (n == 11) { N = 11; return team16_solve_kernel<11>; }
+    if (n <= 16) { N = 16; return team16_solve_kernel<16>; }
     return nullptr;
 }
 
@@ -155,6 +168,9 @@ static int enqueue(BatchedState* b, const SolveArgs& A, hipStream_t s, const Pla
     return GFC_OK;
 }
 
+// Batch size from which the default shape puts four cells in a wavefront.
+constexpr int kTeam16MinCells = 16384;
+
 int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
                    bool graph) {
     auto* b = static_cast<BatchedState*>(ev->batched);
@@ -168,8 +184,20 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
     }
     int N = 0;
     Plan pl;
-    pl.team = team_kernel_for(n, N);
-    if (pl.team && ev->dmTeam.teamWidth <= kTeamWidth) {
+    // Four cells per wavefront when asked for, and by default for small
+    // mechanisms once the batch fills the chip four times over: on MI355X it
+    // takes 0.67x the time of the one-cell form at 65536 H2 cells and 1.09x
+    // at 4096, where 1024 wavefronts leave most SIMDs idle.
+    const bool want16 = ev->cellsPerWave == 4 ||
+                        (ev->cellsPerWave == 0 && A.nCells >= kTeam16MinCells);
+    KernelFn t16 = want16 ? team16_kernel_for(ev->dmTeam, N) : nullptr;
+    pl.team = t16 ? t16 : team_kernel_for(n, N);
+    if (t16) {
+        int perCU = resident_blocks(t16);
+        if (ev->wavesPerBlock > 0 && ev->wavesPerBlock < perCU) perCU = ev->wavesPerBlock;
+        pl.grid = b->numCUs * perCU;
+        pl.key = 2000 + N;
+    } else if (pl.team && ev->dmTeam.teamWidth <= kTeamWidth) {
         int perCU = resident_blocks(pl.team);
         if (ev->wavesPerBlock > 0 && ev->wavesPerBlock < perCU) perCU = ev->wavesPerBlock;
         pl.grid = b->numCUs * perCU;
@@ -188,7 +216,12 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
     }
     SolveArgs At = A;
     At.m = ev->dmTeam;
-    if (pl.grid > A.nCells) pl.grid = A.nCells > 0 ? A.nCells : 1;
+    if (t16) {
+        const int waves = (A.nCells + 3) / 4;
+        if (pl.grid > waves) pl.grid = waves > 0 ? waves : 1;
+    } else if (pl.grid > A.nCells) {
+        pl.grid = A.nCells > 0 ? A.nCells : 1;
+    }
     const int grid = pl.grid;
     if (graph) {
         // inside a caller's own capture the launches simply join that graph
@@ -245,11 +278,21 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
 extern "C" {
 int gfc_evaluator_set_batched_shape(gfc_evaluator* ev, int cellsPerWave,
                                     int wavesPerBlock, int chunk) {
-    if (!ev || (cellsPerWave != 0 && cellsPerWave != 1 && cellsPerWave != 64) ||
+    if (!ev || (cellsPerWave != 0 && cellsPerWave != 1 && cellsPerWave != 4 &&
+                cellsPerWave != 64) ||
         wavesPerBlock < 0 || chunk < 0 || chunk > 1) {
         gfc::setLastError("gfc_evaluator_set_batched_shape: unsupported shape "
-                          "(cellsPerWave must be 0, 1 or 64, chunk 0 or 1)");
+                          "(cellsPerWave must be 0, 1, 4 or 64, chunk 0 or 1)");
         return GFC_ERR_INVALID_ARGUMENT;
+    }
+    if (cellsPerWave == 4) {
+        int N = 0;
+        if (!gfc::team16_kernel_for(ev->dmTeam, N)) {
+            gfc::setLastError("gfc_evaluator_set_batched_shape: 4 cells per wavefront "
+                              "needs nSpecie+1 <= 16, at most 16 third-body efficiency "
+                              "sets and at most 3 species per reaction side");
+            return GFC_ERR_INVALID_ARGUMENT;
+        }
     }
     ev->cellsPerWave = cellsPerWave;
     ev->wavesPerBlock = wavesPerBlock;

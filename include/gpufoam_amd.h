@@ -202,8 +202,12 @@ int gfc_solve(gfc_evaluator* ev, double deltaT, double deltaTChemMax,
 
 /* Launch-shape control for the persistent batched form (the autotuner's
  * search space, gpufoam_amd/autotune.py).  cellsPerWave: 0 = automatic
- * (one wavefront per cell where the mechanism fits, else one lane per
- * cell), 1 = one wavefront per cell, 64 = one lane per cell.
+ * (by mechanism size: four cells per wavefront for nSpecie+1 <= 16 from
+ * 16384 cells on, one wavefront per cell up to 64, one 512-thread workgroup
+ * per cell up to 208, else one lane per cell), 1 = one wavefront per cell,
+ * 4 = four 16-lane cells per wavefront (nSpecie+1 <= 16, at most 16
+ * third-body efficiency sets, at most 3 species per reaction side; other
+ * mechanisms return GFC_ERR_INVALID_ARGUMENT), 64 = one lane per cell.
  * wavesPerBlock: resident wavefronts per compute unit of the persistent
  * grid (0 = as many as fit).  chunk: cells taken per queue grab (0 or 1;
  * larger values are reserved). */

@@ -36,7 +36,14 @@ def test_autotune_sets_the_fastest_shape_and_keeps_results(config):
 def test_invalid_shapes_are_rejected():
     m, Y0, dt0, c = make_states("h2_small", 8)
     ev = Evaluator(m, ode_inputs(c["solver"]), 0.0, max_cells=8)
-    for bad in ((4, 0, 0), (16, 0, 0), (1, -1, 0), (1, 0, 8)):
+    for bad in ((2, 0, 0), (16, 0, 0), (1, -1, 0), (1, 0, 8)):
         with pytest.raises(GfcError) as e:
             ev.set_batched_shape(*bad)
         assert e.value.status == 1
+    ev.set_batched_shape(4, 0)  # 16-lane teams: H2 has n = 11
+    # four cells per wavefront needs n <= 16: not GRI
+    m, Y0, dt0, c = make_states("gri_production", 8)
+    ev = Evaluator(m, ode_inputs(c["solver"]), 0.0, max_cells=8)
+    with pytest.raises(GfcError) as e:
+        ev.set_batched_shape(4, 0)
+    assert e.value.status == 1

@@ -124,6 +124,40 @@ def test_other_solvers(solver, batched):
     _check(m, got, ref, tight_dt=not batched)
 
 
+@pytest.mark.parametrize("cfg,n", [("h2_small", 384), ("h2_adjoint", 96)])
+def test_four_cells_per_wavefront_matches_oracle(cfg, n):
+    """The 16-lane-team form (four cells per wavefront, selected with
+    gfc_evaluator_set_batched_shape(4, ...)) on H2 states; cells of a
+    wavefront take independent step sequences."""
+    from gpufoam_amd.evaluator import Evaluator
+    m, Y0, dt0, c, ode, rho, ref = _oracle_run(cfg, n)
+    ev = Evaluator(m, ode, 0.0, max_cells=n)
+    ev.set_batched_shape(4, 0)
+    print(f"case {cfg} n={n} cells per wavefront 4", flush=True)
+    dY0 = torch.tensor(Y0, device="cuda")
+    dYn = torch.empty_like(dY0)
+    dRR = torch.zeros((m.n_species, n), dtype=torch.float64, device="cuda")
+    ddt = torch.tensor(dt0, device="cuda")
+    dmin = torch.zeros(1, dtype=torch.float64, device="cuda")
+    st = torch.zeros((3, n), dtype=torch.int32, device="cuda")
+    ev.solve(c["deltaT"], dY0, ddt, Ynew=dYn, RR=dRR, rho=torch.tensor(rho, device="cuda"),
+             deltaTMin=dmin, stats=st, batched=True)
+    torch.cuda.synchronize()
+    got = (dYn.cpu().numpy(), ddt.cpu().numpy(), dmin.item(), st.cpu().numpy(),
+           dRR.cpu().numpy())
+    _check(m, got, ref, tight_dt=False, Y0=Y0, rho=rho, deltaT=c["deltaT"])
+    # repeated launches and graph replays give the same bits
+    outs = []
+    for graph in (False, True, True):
+        Yn = torch.empty_like(dY0)
+        d = torch.tensor(dt0, device="cuda")
+        ev.solve(c["deltaT"], dY0, d, Ynew=Yn, batched=True, graph=graph)
+        torch.cuda.synchronize()
+        outs.append((Yn.cpu().numpy(), d.cpu().numpy()))
+    for o in outs:
+        assert np.array_equal(o[0], got[0]) and np.array_equal(o[1], got[1])
+
+
 def test_batched_is_deterministic_and_graph_replays():
     m, Y0, dt0, c = make_states("gri_production", 96)
     ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])

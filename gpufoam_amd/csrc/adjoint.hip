@@ -28,10 +28,23 @@
 #include "adjoint_math.hpp"
 #include "evaluator.hpp"
 #include "mechanism.hpp"
+#include "team16_common.hpp"
 
 namespace gfc {
 
+// Four cells per wavefront (team16_kernel.hip, team16_adjoint.hip): used
+// when the batched solve would use that shape (nSpecie+1 <= 16).
+template <int N>
+__global__ void team16_record_kernel(SolveArgs A, int* queue, t16::T16Record R);
+template <int N>
+__global__ void team16_adjoint_reverse_kernel(SolveArgs A, DeviceMech mr, t16::T16Record R,
+                                              double* seg, long segPer, const double* Ybar,
+                                              long ldb, double* Y0bar, double* thBar,
+                                              int* queue);
+
 namespace {
+
+__global__ void adj_reset_queue_kernel(int* q) { *q = 0; }
 
 bool hipFail(hipError_t e, const char* what) {
     if (e == hipSuccess) return false;
@@ -51,15 +64,18 @@ struct AdjState {
     double* y0 = nullptr;
     double* yEnd = nullptr;
     int* nAcc = nullptr;
+    int* queue = nullptr; // work queue of the four-cells-per-wavefront kernels
     int nCells = 0;
     double deltaT = 0;
     bool valid = false;
+    int teamN = 0;        // > 0: recorded by the four-cells-per-wavefront kernel
+    int numCUs = 0;
     void release() {
-        void* ptrs[] = {scratch, ck, hs, y0, yEnd, nAcc};
+        void* ptrs[] = {scratch, ck, hs, y0, yEnd, nAcc, queue};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         scratch = ck = hs = y0 = yEnd = nullptr;
-        nAcc = nullptr;
+        nAcc = queue = nullptr;
         cap = 0;
         valid = false;
     }
@@ -226,11 +242,38 @@ static int adjoint_reserve(gfc_evaluator* ev, int nCells) {
         hipFail(hipMalloc(&st->hs, (size_t)8 * maxSteps * cap), "hipMalloc(step sizes)") ||
         hipFail(hipMalloc(&st->y0, (size_t)8 * (n + 1) * cap), "hipMalloc(adjoint y0)") ||
         hipFail(hipMalloc(&st->yEnd, (size_t)8 * n * cap), "hipMalloc(adjoint yEnd)") ||
-        hipFail(hipMalloc(&st->nAcc, (size_t)4 * cap), "hipMalloc(adjoint counts)")) {
+        hipFail(hipMalloc(&st->nAcc, (size_t)4 * cap), "hipMalloc(adjoint counts)") ||
+        hipFail(hipMalloc(&st->queue, sizeof(int)), "hipMalloc(adjoint queue)")) {
         st->release();
         return GFC_ERR_HIP;
     }
     return GFC_OK;
+}
+
+static t16::T16Record record_of(const AdjState* st, const AdjLayout& W) {
+    return t16::T16Record{st->ck, st->hs, st->nAcc, st->yEnd, st->y0,
+                          W.nck, W.maxSteps, W.K};
+}
+
+// Persistent grid of a four-cells-per-wavefront kernel: the wavefronts that
+// stay resident, at most one per four cells.
+template <class K>
+static int team16_grid(AdjState* st, K kernel, int nCells) {
+    if (st->numCUs == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&st->numCUs, hipDeviceAttributeMultiprocessorCount, dev) !=
+                hipSuccess ||
+            st->numCUs <= 0)
+            st->numCUs = 1;
+    }
+    int perCU = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void*)kernel, 64, 0) !=
+            hipSuccess ||
+        perCU <= 0)
+        perCU = 1;
+    const long waves = ((long)nCells + 3) / 4;
+    return (int)std::max(1L, std::min<long>((long)st->numCUs * perCU, waves));
 }
 
 } // namespace gfc
@@ -328,9 +371,21 @@ int gfc_adjoint_forward(gfc_evaluator* ev, double deltaT, int nCells, int ldY,
     A.Ynew = Ynew;
     A.dtChem = deltaTChem;
     A.stats = cellStats;
-    const int blocks = (nCells + 63) / 64;
-    hipLaunchKernelGGL(adjoint_forward_kernel, dim3(blocks), dim3(64), 0, s, A, layout(ev),
-                       st->scratch, st->ck, st->hs, st->nAcc, st->yEnd, st->y0);
+    int N = 0;
+    st->teamN = team16_selected(ev, nCells, N) ? N : 0;
+    if (st->teamN) {
+        SolveArgs At = A;
+        At.m = ev->dmTeam;
+        const t16::T16Record R = record_of(st, layout(ev));
+        auto k = (N == 11) ? team16_record_kernel<11> : team16_record_kernel<16>;
+        hipLaunchKernelGGL(adj_reset_queue_kernel, dim3(1), dim3(1), 0, s, st->queue);
+        hipLaunchKernelGGL(k, dim3(team16_grid(st, k, nCells)), dim3(64), 0, s, At, st->queue,
+                           R);
+    } else {
+        const int blocks = (nCells + 63) / 64;
+        hipLaunchKernelGGL(adjoint_forward_kernel, dim3(blocks), dim3(64), 0, s, A, layout(ev),
+                           st->scratch, st->ck, st->hs, st->nAcc, st->yEnd, st->y0);
+    }
     if (hipFail(hipGetLastError(), "adjoint forward launch")) return GFC_ERR_HIP;
     st->nCells = nCells;
     st->deltaT = deltaT;
@@ -352,10 +407,24 @@ int gfc_adjoint_backward(gfc_evaluator* ev, const double* Ybar, int ldb, double*
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
     SolveArgs A = solve_args(ev, st->deltaT, 1e300, st->nCells);
-    const int blocks = (st->nCells + 63) / 64;
-    hipLaunchKernelGGL(adjoint_reverse_kernel, dim3(blocks), dim3(64), 0, s, A, layout(ev),
-                       st->scratch, st->ck, st->hs, st->nAcc, st->yEnd, st->y0, Ybar,
-                       (long)ldb, Y0bar, paramBar);
+    const AdjLayout W = layout(ev);
+    if (st->teamN) {
+        SolveArgs At = A;
+        At.m = ev->dmTeam;
+        auto k = (st->teamN == 11) ? team16_adjoint_reverse_kernel<11>
+                                   : team16_adjoint_reverse_kernel<16>;
+        // the segment buffers sit after the forward and adjoint scratch
+        double* seg = st->scratch + (W.laneD + W.adjD) * lane::kSlotStride;
+        hipLaunchKernelGGL(adj_reset_queue_kernel, dim3(1), dim3(1), 0, s, st->queue);
+        hipLaunchKernelGGL(k, dim3(team16_grid(st, k, st->nCells)), dim3(64), 0, s, At, ev->dm,
+                           record_of(st, W), seg, W.perScratch, Ybar, (long)ldb, Y0bar,
+                           paramBar, st->queue);
+    } else {
+        const int blocks = (st->nCells + 63) / 64;
+        hipLaunchKernelGGL(adjoint_reverse_kernel, dim3(blocks), dim3(64), 0, s, A, W,
+                           st->scratch, st->ck, st->hs, st->nAcc, st->yEnd, st->y0, Ybar,
+                           (long)ldb, Y0bar, paramBar);
+    }
     if (hipFail(hipGetLastError(), "adjoint reverse launch")) return GFC_ERR_HIP;
     return GFC_OK;
 }

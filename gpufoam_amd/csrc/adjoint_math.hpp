@@ -241,48 +241,55 @@ GFC_HD __forceinline__ double specie_pow_m2(double x, int ei, double er) {
     return pow(x, er - 2.0);
 }
 
+// Accumulation into an adjoint vector (one lane per cell: a plain add; a
+// team of lanes sharing one cell supplies an atomic overload).
+GFC_HD __forceinline__ void acc_add(SV v, int i, double x) { v[i] += x; }
+
 // One side of a reaction: C = prod c^e, dC[j] = dC/dc_j (product without j).
-struct Side {
+// W bounds the number of species on a side.
+template <int W>
+struct SideT {
     int n;
     const int* idx;
     const int* ei;
     const double* er;
-    double pw[kWidth], d1[kWidth], dC[kWidth];
+    double pw[W], d1[W], dC[W];
     double C;
     GFC_HD void eval(SV c) {
-        for (int q = 0; q < n; ++q) {
+        for (int q = 0; q < W && q < n; ++q) {
             pw[q] = specie_pow(c[idx[q]], ei[q], er[q]);
             d1[q] = er[q] * specie_pow_m1(c[idx[q]], ei[q], er[q]);
         }
         C = 1.0;
-        for (int q = 0; q < n; ++q) C *= pw[q];
-        for (int j = 0; j < n; ++j) {
+        for (int q = 0; q < W && q < n; ++q) C *= pw[q];
+        for (int j = 0; j < W && j < n; ++j) {
             double v = 1;
-            for (int i = 0; i < n; ++i) v *= (i == j) ? d1[i] : pw[i];
+            for (int i = 0; i < W && i < n; ++i) v *= (i == j) ? d1[i] : pw[i];
             dC[j] = v;
         }
     }
     // cbar[idx[l]] += Cb dC/dc_l + sum_j dCb[j] d2C/dc_j dc_l
-    GFC_HD void reverse(SV c, double Cb, const double* dCb, SV cbar) const {
-        for (int l = 0; l < n; ++l) {
+    template <class CB>
+    GFC_HD void reverse(SV c, double Cb, const double* dCb, CB cbar) const {
+        for (int l = 0; l < W && l < n; ++l) {
             double acc = Cb * dC[l];
-            for (int j = 0; j < n; ++j) {
+            for (int j = 0; j < W && j < n; ++j) {
                 if (dCb[j] == 0.0) continue;
                 double h;
                 if (j == l) {
                     const double coef = er[j] * (er[j] - 1.0);
                     if (coef == 0.0) continue;
                     h = coef * specie_pow_m2(c[idx[j]], ei[j], er[j]);
-                    for (int q = 0; q < n; ++q)
+                    for (int q = 0; q < W && q < n; ++q)
                         if (q != j) h *= pw[q];
                 } else {
                     h = d1[j] * d1[l];
-                    for (int q = 0; q < n; ++q)
+                    for (int q = 0; q < W && q < n; ++q)
                         if (q != j && q != l) h *= pw[q];
                 }
                 acc += dCb[j] * h;
             }
-            cbar[idx[l]] += acc;
+            acc_add(cbar, idx[l], acc);
         }
     }
 };
@@ -290,18 +297,21 @@ struct Side {
 // Adjoints of the quantities one reaction contributes: omega, and for the
 // Jacobian its fast dOmega/dT, kf*dCf/dc_j, kr*dCr/dc_j and the third-body
 // scalar D (ddc_j = eff_j*D).
-struct RxnBar {
+template <int W>
+struct RxnBarT {
     double w = 0, dwdT = 0, D = 0;
-    double dCf[kWidth], dCr[kWidth];
+    double dCf[W], dCr[W];
     bool fast = false;
 };
+using RxnBar = RxnBarT<kWidth>;
 
 // Reverse of reaction r at temperature T (clipped to [Tlow, Thigh] when
 // `clip`, as gpuReaction::omega does; the Jacobian uses T unclipped).
-// Adds to cbar, Tb and th[0..5].
-GFC_HD __forceinline__ void reaction_reverse(const DeviceMech& m, int r, double T, bool clip,
-                                          SV c, const RxnBar& in, SV cbar, double& Tb,
-                                          double* th) {
+// Adds to cbar, Tb and th[0..5].  W bounds the species per reaction side.
+template <int W, class CB>
+GFC_HD __forceinline__ void reaction_reverse_w(const DeviceMech& m, int r, double T, bool clip,
+                                            SV c, const RxnBarT<W>& in, CB cbar, double& Tb,
+                                            double* th) {
     const double Te = clip ? fmin(fmax(T, m.Tlow), m.Thigh) : T;
     const double lnT = log(Te);
     const int t = m.type[r];
@@ -333,9 +343,9 @@ GFC_HD __forceinline__ void reaction_reverse(const DeviceMech& m, int r, double 
         kf = kI * temp;
         dkfdT = (kI * qI) * temp;
     }
-    Side L{m.nl[r], m.li + o, m.lei + o, m.le + o};
+    SideT<W> L{m.nl[r], m.li + o, m.lei + o, m.le + o};
     L.eval(c);
-    Side R{m.nr[r], m.ri + o, m.rei + o, m.re + o};
+    SideT<W> R{m.nr[r], m.ri + o, m.rei + o, m.re + o};
     double Cr = 0.0;
     KcVal K{1.0, 0.0, 0.0, 0.0};
     if (rev) {
@@ -349,19 +359,19 @@ GFC_HD __forceinline__ void reaction_reverse(const DeviceMech& m, int r, double 
 
     double Cfb = in.w * kf, Crb = -in.w * kr, kfb = in.w * Cf, krb = -in.w * Cr;
     double dkfdTb = 0.0, dkrdTb = 0.0, Kcb = 0.0, kAb = 0.0;
-    double dCLb[kWidth], dCRb[kWidth];
-    for (int j = 0; j < kWidth; ++j) dCLb[j] = dCRb[j] = 0.0;
+    double dCLb[W], dCRb[W];
+    for (int j = 0; j < W; ++j) dCLb[j] = dCRb[j] = 0.0;
     if (in.fast) {
         Cfb += in.dwdT * dkfdT;
         Crb += -in.dwdT * dkrdT;
         dkfdTb = in.dwdT * Cf;
         dkrdTb = -in.dwdT * Cr;
-        for (int j = 0; j < L.n; ++j) {
+        for (int j = 0; j < W && j < L.n; ++j) {
             kfb += in.dCf[j] * L.dC[j];
             dCLb[j] = in.dCf[j] * kf;
         }
         if (rev)
-            for (int j = 0; j < R.n; ++j) {
+            for (int j = 0; j < W && j < R.n; ++j) {
                 krb += in.dCr[j] * R.dC[j];
                 dCRb[j] = in.dCr[j] * kr;
             }
@@ -404,8 +414,14 @@ GFC_HD __forceinline__ void reaction_reverse(const DeviceMech& m, int r, double 
         arr_adjoint(ki, Te, lnT, kI, kIb, kqIb, TbE, th + 3);
     }
     if (Mb != 0.0)
-        for (int i = 0; i < m.nS; ++i) cbar[i] += eff[i] * Mb;
+        for (int i = 0; i < m.nS; ++i) acc_add(cbar, i, eff[i] * Mb);
     if (!clip || (T >= m.Tlow && T <= m.Thigh)) Tb += TbE;
+}
+
+GFC_HD __forceinline__ void reaction_reverse(const DeviceMech& m, int r, double T, bool clip,
+                                          SV c, const RxnBar& in, SV cbar, double& Tb,
+                                          double* th) {
+    reaction_reverse_w<kWidth>(m, r, T, clip, c, in, cbar, Tb, th);
 }
 
 // ---------------------------------------------------- right-hand side --

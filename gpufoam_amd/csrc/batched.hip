@@ -171,6 +171,16 @@ static int enqueue(BatchedState* b, const SolveArgs& A, hipStream_t s, const Pla
 // Batch size from which the default shape puts four cells in a wavefront.
 constexpr int kTeam16MinCells = 16384;
 
+// Four cells per wavefront when asked for, and by default for small
+// mechanisms once the batch fills the chip four times over: on MI355X the
+// batched solve takes 0.39x the time of the one-cell form at 65536 H2 cells
+// and 1.09x at 4096, where 1024 wavefronts leave most SIMDs idle.
+bool team16_selected(const gfc_evaluator* ev, int nCells, int& N) {
+    const bool want = ev->cellsPerWave == 4 ||
+                      (ev->cellsPerWave == 0 && nCells >= kTeam16MinCells);
+    return want && team16_kernel_for(ev->dmTeam, N) != nullptr;
+}
+
 int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
                    bool graph) {
     auto* b = static_cast<BatchedState*>(ev->batched);
@@ -184,13 +194,8 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
     }
     int N = 0;
     Plan pl;
-    // Four cells per wavefront when asked for, and by default for small
-    // mechanisms once the batch fills the chip four times over: on MI355X it
-    // takes 0.67x the time of the one-cell form at 65536 H2 cells and 1.09x
-    // at 4096, where 1024 wavefronts leave most SIMDs idle.
-    const bool want16 = ev->cellsPerWave == 4 ||
-                        (ev->cellsPerWave == 0 && A.nCells >= kTeam16MinCells);
-    KernelFn t16 = want16 ? team16_kernel_for(ev->dmTeam, N) : nullptr;
+    KernelFn t16 = team16_selected(ev, A.nCells, N) ? team16_kernel_for(ev->dmTeam, N)
+                                                    : nullptr;
     pl.team = t16 ? t16 : team_kernel_for(n, N);
     if (t16) {
         int perCU = resident_blocks(t16);

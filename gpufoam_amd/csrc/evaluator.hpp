@@ -51,4 +51,7 @@ int batched_init(gfc_evaluator* ev);
 void batched_free(gfc_evaluator* ev);
 int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
                    bool graph);
+// Whether a launch of nCells cells runs four cells per wavefront (the
+// evaluator's shape, or the automatic choice); N: the kernels' size.
+bool team16_selected(const gfc_evaluator* ev, int nCells, int& N);
 } // namespace gfc

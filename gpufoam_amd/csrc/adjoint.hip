@@ -37,10 +37,9 @@ namespace gfc {
 template <int N>
 __global__ void team16_record_kernel(SolveArgs A, int* queue, t16::T16Record R);
 template <int N>
-__global__ void team16_adjoint_reverse_kernel(SolveArgs A, DeviceMech mr, t16::T16Record R,
-                                              double* seg, long segPer, const double* Ybar,
-                                              long ldb, double* Y0bar, double* thBar,
-                                              int* queue);
+__global__ void team16_adjoint_reverse_kernel(SolveArgs A, t16::T16Record R, double* seg,
+                                              long segPer, const double* Ybar, long ldb,
+                                              double* Y0bar, double* thBar, int* queue);
 
 namespace {
 
@@ -416,7 +415,7 @@ int gfc_adjoint_backward(gfc_evaluator* ev, const double* Ybar, int ldb, double*
         // the segment buffers sit after the forward and adjoint scratch
         double* seg = st->scratch + (W.laneD + W.adjD) * lane::kSlotStride;
         hipLaunchKernelGGL(adj_reset_queue_kernel, dim3(1), dim3(1), 0, s, st->queue);
-        hipLaunchKernelGGL(k, dim3(team16_grid(st, k, st->nCells)), dim3(64), 0, s, At, ev->dm,
+        hipLaunchKernelGGL(k, dim3(team16_grid(st, k, st->nCells)), dim3(64), 0, s, At,
                            record_of(st, W), seg, W.perScratch, Ybar, (long)ldb, Y0bar,
                            paramBar, st->queue);
     } else {

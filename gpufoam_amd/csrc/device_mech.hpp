@@ -88,6 +88,9 @@ struct DeviceMech {
     const double* effSetsT;
     const int* effSet;
     int nSets;
+    // reference index of the reaction stored at r (the identity for the
+    // reference-order tables)
+    const int* rxnRef;
     // wavefront kernel: one record per reaction, field-major within a pass
     // so that the 64 lanes of a pass load each field in one coalesced access:
     // recD[(pass*kRecD + f)*64 + lane], recI[(pass*kRecI + f)*64 + lane]

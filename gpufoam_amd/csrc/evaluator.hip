@@ -256,6 +256,7 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     add(effSets.data(), effSets.size());
     add(effSetsT.data(), effSetsT.size());
     add(effSet.data(), effSet.size());
+    add(perm.data(), perm.size());
     add(recD.data(), recD.size());
     add(recI.data(), recI.size());
     add(spPtr.data(), spPtr.size());
@@ -296,6 +297,7 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     at(d.effSets);
     at(d.effSetsT);
     at(d.effSet);
+    at(d.rxnRef);
     at(d.recD);
     at(d.recI);
     at(d.blk.spPtr); at(d.blk.spR); at(d.blk.spNu);

@@ -18,9 +18,9 @@
 //     atomics for the concentration adjoints;
 //   * team sums are row_shr trees, so sums differ from the one-lane form's
 //     sequential order in the last bits.
-// The reaction sweeps use the mechanism tables in reference order (the
-// rate-parameter adjoints are indexed by reaction); the recomputation uses
-// the type-sorted tables of the team solve.
+// All sweeps use the team solve's tables, reactions grouped by rate type so
+// that the members of a pass mostly take the same branch; rate-parameter
+// adjoints go to each reaction's reference index (rxnRef).
 #include <hip/hip_runtime.h>
 
 #include "adjoint_math.hpp"
@@ -230,7 +230,7 @@ __device__ __forceinline__ void t16_mixture_reverse(const DeviceMech& m, double 
 }
 
 // ybar += d<fbar, f(y)>/dy, th += d<fbar, f(y)>/dtheta (adjoint_math.hpp
-// vjp_f); fy = f(y).  mr: reference-order tables.
+// vjp_f); fy = f(y).
 template <int N>
 __device__ __forceinline__ void t16_vjp_f(const DeviceMech& mr, T16AdjLDS<N>& L, double p,
                                           double y, double fbar, double fy, double& ybar,
@@ -267,7 +267,7 @@ __device__ __forceinline__ void t16_vjp_f(const DeviceMech& mr, T16AdjLDS<N>& L,
         double t6[adj::kParams] = {0, 0, 0, 0, 0, 0};
         adj::reaction_reverse_w<kTeamWidth>(mr, r, T, true, lane::SV{L.f.c, 1}, in,
                                             adj::LdsAcc{L.cb}, Tb, t6);
-        th.add(r, t6);
+        th.add(mr.rxnRef[r], t6);
     }
     Tb = team_sum(Tb, lane);
     wave_sync();
@@ -416,7 +416,7 @@ __device__ __forceinline__ void t16_jact(const DeviceMech& mr, T16AdjLDS<N>& L, 
         double t6[adj::kParams] = {0, 0, 0, 0, 0, 0};
         adj::reaction_reverse_w<kTeamWidth>(mr, r, T, false, lane::SV{L.f.c, 1}, in,
                                             adj::LdsAcc{L.cb}, Tb, t6);
-        th.add(r, t6);
+        th.add(mr.rxnRef[r], t6);
     }
     Tb = team_sum(Tb, lane);
     wave_sync();
@@ -497,18 +497,19 @@ __device__ __forceinline__ double t16_step_reverse(const SolveArgs& A, const Dev
 } // namespace
 
 // Reverse sweep, one team per cell (adjoint.hip adjoint_reverse_kernel).
-// A.m: the team solve's tables; mr: reference order.  seg: the K-state
-// segment buffers (segPer doubles per cell, wave-blocked).  Ybar/Y0bar:
-// [n][ldb]; thBar: [6 nR][ldb].
+// A.m: the team solve's tables.  seg: the K-state segment buffers (segPer
+// doubles per cell, wave-blocked).  Ybar/Y0bar: [n][ldb]; thBar:
+// [6 nR][ldb], reactions in reference order.
 template <int N>
 __global__ void __launch_bounds__(64)
-team16_adjoint_reverse_kernel(SolveArgs A, DeviceMech mr, T16Record R, double* seg,
-                              long segPer, const double* Ybar, long ldb, double* Y0bar,
-                              double* thBar, int* queue) {
+team16_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer,
+                              const double* Ybar, long ldb, double* Y0bar, double* thBar,
+                              int* queue) {
     __shared__ T16AdjLDS<N> Ls[kLanes / kTW];
     const int lane = threadIdx.x;
     const int team = lane >> 4, mem = lane & 15;
     T16AdjLDS<N>& L = Ls[team];
+    const DeviceMech& mr = A.m;
     const int nS = A.m.nS, n = nS + 1;
     for (;;) {
         int c = 0;
@@ -551,11 +552,11 @@ team16_adjoint_reverse_kernel(SolveArgs A, DeviceMech mr, T16Record R, double* s
     }
 }
 
-template __global__ void team16_adjoint_reverse_kernel<11>(SolveArgs, DeviceMech, T16Record,
-                                                           double*, long, const double*, long,
-                                                           double*, double*, int*);
-template __global__ void team16_adjoint_reverse_kernel<16>(SolveArgs, DeviceMech, T16Record,
-                                                           double*, long, const double*, long,
-                                                           double*, double*, int*);
+template __global__ void team16_adjoint_reverse_kernel<11>(SolveArgs, T16Record, double*, long,
+                                                           const double*, long, double*,
+                                                           double*, int*);
+template __global__ void team16_adjoint_reverse_kernel<16>(SolveArgs, T16Record, double*, long,
+                                                           const double*, long, double*,
+                                                           double*, int*);
 
 } // namespace gfc

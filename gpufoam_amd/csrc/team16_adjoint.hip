@@ -43,14 +43,21 @@ using namespace t16;
 template <int N>
 struct T16AdjLDS {
     static constexpr int P = Team16LDS<N>::P;
+    // The LU factors and Jbar live in the solve's dense third-body weights
+    // (f.Wd) where they fit: the weights are used only inside t16_jacobian,
+    // the factors and Jbar only after it, until the step's reverse ends.
+    static constexpr bool kShare = 2 * N * P <= kSets16 * kTW;
     Team16LDS<N> f;     // the solve's arrays; f.J holds the step's Jacobian
     double raw[N * P];  // species rows before the transform
-    double LU[N * P];   // LU factors, row j = pivot position j
-    double Jb[N * P];   // adjoint of the Jacobian
+    double own[kShare ? 1 : 2 * N * P];
     double cb[kTW];     // concentration adjoints
     double dNb[kTW];    // adjoints of dN_i/dt
     double vj[kTW];     // 1/rho_i of each specie
     double v[kTW];      // a vector being broadcast
+    // LU factors, row j = pivot position j; then the adjoint of the Jacobian
+    __device__ __forceinline__ double* LU() { return kShare ? f.Wd : own; }
+    __device__ __forceinline__ const double* LU() const { return kShare ? f.Wd : own; }
+    __device__ __forceinline__ double* Jb() { return LU() + N * P; }
 };
 
 // One step's recomputed quantities, this member's components.
@@ -61,7 +68,7 @@ struct T16Stages {
 
 // The accepted attempt of a step of size dx from y, as the solve computes it
 // (team16_kernel.hip): returns the new state.  With KEEP, also leaves the
-// pre-transform rows in L.raw and the LU in L.LU.
+// pre-transform rows in L.raw and the LU in L.LU().
 template <int N, bool KEEP>
 __device__ __forceinline__ double t16_step(const SolveArgs& A, T16AdjLDS<N>& L, double p,
                                            double y, double dx, int lane, int mem,
@@ -92,7 +99,7 @@ __device__ __forceinline__ double t16_step(const SolveArgs& A, T16AdjLDS<N>& L, 
         if (mem < n && ps < n) {
 #pragma unroll
             for (int k = 0; k < N; ++k)
-                if (k < n) L.LU[ps * P + k] = row[k];
+                if (k < n) L.LU()[ps * P + k] = row[k];
         }
     }
     double k1 = 0, k2 = 0, k3 = 0, k4 = 0, dydx = 0, f2 = 0, f3 = 0;
@@ -139,16 +146,16 @@ __device__ __forceinline__ double t16_solveT(const T16AdjLDS<N>& L, int n, int p
     // U^T w = r
     double w = 0.0;
     for (int j = 0; j < n; ++j) {
-        const double wj = team_bcast(r, lane, j) / L.LU[j * P + j];
+        const double wj = team_bcast(r, lane, j) / L.LU()[j * P + j];
         if (mem == j) w = wj;
-        if (mem > j && mem < n) r -= L.LU[j * P + mem] * wj;
+        if (mem > j && mem < n) r -= L.LU()[j * P + mem] * wj;
     }
     // L^T v = w (unit diagonal)
     double v = 0.0;
     for (int k = n - 1; k >= 0; --k) {
         const double vk = team_bcast(w, lane, k);
         if (mem == k) v = vk;
-        if (mem < k) w -= L.LU[k * P + mem] * vk;
+        if (mem < k) w -= L.LU()[k * P + mem] * vk;
     }
     // x = P^T v: row i sits at position ps_i
     return team_bcast(v, lane, ps);
@@ -162,7 +169,7 @@ __device__ __forceinline__ void t16_add_outer(T16AdjLDS<N>& L, int n, double r, 
     L.v[mem] = k;
     wave_sync();
     if (mem < n && r != 0.0)
-        for (int j = 0; j < n; ++j) L.Jb[mem * P + j] += r * L.v[j];
+        for (int j = 0; j < n; ++j) L.Jb()[mem * P + j] += r * L.v[j];
     wave_sync();
 }
 
@@ -298,11 +305,11 @@ __device__ __forceinline__ void t16_jact(const DeviceMech& mr, T16AdjLDS<N>& L, 
     adj::MixBar mb;
     // ddT_dtdT (gpuODESystem.H:129-146)
     {
-        const double g = L.Jb[iT * P + iT];
+        const double g = L.Jb()[iT * P + iT];
         if (sp) {
             Fb += -g * x.Cp / CpM;
             Cpb += -g * F / CpM;
-            L.Jb[mem * P + iT] += -g * x.Ha / CpM;
+            L.Jb()[mem * P + iT] += -g * x.Ha / CpM;
             Hab += -g * L.f.J[mem * P + iT] / CpM;
         }
         if (mem == iT) Fb += -g * x.dCpMdT / CpM;
@@ -314,13 +321,13 @@ __device__ __forceinline__ void t16_jact(const DeviceMech& mr, T16AdjLDS<N>& L, 
     {
         if (sp) {
             for (int i = 0; i < nS; ++i) {
-                const double g = L.Jb[iT * P + i];
+                const double g = L.Jb()[iT * P + i];
                 if (g == 0.0) continue;
-                L.Jb[mem * P + i] += -g * x.Ha / CpM;
+                L.Jb()[mem * P + i] += -g * x.Ha / CpM;
                 Hab += -g * L.f.J[mem * P + i] / CpM;
             }
         }
-        const double gi = sp ? L.Jb[iT * P + mem] : 0.0;
+        const double gi = sp ? L.Jb()[iT * P + mem] : 0.0;
         double fbT = 0.0, cpm = 0.0;
         if (sp && gi != 0.0) {
             Cpb += -gi * FT / CpM;
@@ -337,8 +344,8 @@ __device__ __forceinline__ void t16_jact(const DeviceMech& mr, T16AdjLDS<N>& L, 
     // concentration adjoints.
     if (sp) {
         for (int i = 0; i < nS; ++i) {
-            vb += L.Jb[i * P + mem] * rho * L.v[i];
-            const double g = L.Jb[i * P + iT];
+            vb += L.Jb()[i * P + mem] * rho * L.v[i];
+            const double g = L.Jb()[i * P + iT];
             cbT += -g * (mr.W[i] / rho) * L.raw[i * P + mem] * av;
         }
     }
@@ -346,7 +353,7 @@ __device__ __forceinline__ void t16_jact(const DeviceMech& mr, T16AdjLDS<N>& L, 
     double rhoPart = 0.0, avPart = 0.0;
     if (sp) {
         const double WiR = mr.W[mem] / rho;
-        double* row = &L.Jb[mem * P];
+        double* row = &L.Jb()[mem * P];
         for (int j = 0; j < nS; ++j) {
             const double g = row[j];
             rhoPart += g * L.vj[j] * F;
@@ -397,8 +404,8 @@ __device__ __forceinline__ void t16_jact(const DeviceMech& mr, T16AdjLDS<N>& L, 
         in.fast = true;
         auto u = [&](int col) {
             double v = 0.0;
-            for (int q = 0; q < kTeamWidth && q < nl; ++q) v -= mr.ls[o + q] * L.Jb[mr.li[o + q] * P + col];
-            for (int q = 0; q < kTeamWidth && q < nr; ++q) v += mr.rs[o + q] * L.Jb[mr.ri[o + q] * P + col];
+            for (int q = 0; q < kTeamWidth && q < nl; ++q) v -= mr.ls[o + q] * L.Jb()[mr.li[o + q] * P + col];
+            for (int q = 0; q < kTeamWidth && q < nr; ++q) v += mr.rs[o + q] * L.Jb()[mr.ri[o + q] * P + col];
             return v;
         };
         for (int q = 0; q < kTeamWidth && q < nl; ++q) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
@@ -442,7 +449,7 @@ __device__ __forceinline__ double t16_step_reverse(const SolveArgs& A, const Dev
     (void)t16_step<N, true>(A, L, p, y0, h, lane, mem, S);
     double kb1 = t.b1 * yb, kb2 = t.b2 * yb, kb3 = t.b3 * yb, kb4 = t.b4 * yb;
     double Fb = 0.0, fb2 = 0.0, fb3 = 0.0;
-    for (int e = mem; e < N * T16AdjLDS<N>::P; e += kTW) L.Jb[e] = 0.0;
+    for (int e = mem; e < N * T16AdjLDS<N>::P; e += kTW) L.Jb()[e] = 0.0;
     wave_sync();
     if (t.stages == 4) {
         const double rb = t16_solveT<N>(L, n, S.ps, kb4, lane, mem);
@@ -501,7 +508,7 @@ __device__ __forceinline__ double t16_step_reverse(const SolveArgs& A, const Dev
 // doubles per cell, wave-blocked).  Ybar/Y0bar: [n][ldb]; thBar:
 // [6 nR][ldb], reactions in reference order.
 template <int N>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, 2)
 team16_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer,
                               const double* Ybar, long ldb, double* Y0bar, double* thBar,
                               int* queue) {

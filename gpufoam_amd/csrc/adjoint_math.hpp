@@ -220,19 +220,12 @@ struct KcVal {
 };
 GFC_HD inline KcVal kc_eval(const DeviceMech& m, int r, double T) {
     KcVal v;
-    reaction_Kc_dKc(m, r, T, v.Kc, v.dK);
-    const double Y = m.rY[r], W = m.rW[r];
-    const double* a = coeffs_of(m.rhigh + 7 * r, m.rlow + 7 * r, m.rTcommon[r], T);
-    const double nm = Y / W;
-    const double logT = log(T);
-    const double Gstd = poly_Gstd(a, T, logT);
-    const double S = poly_S(a, T, logT);
-    const double RRT = kRR * T;
-    const bool nmSmall = fabs(nm - kSmall) < kVSmall;
-    const double HT = S + Gstd / T; // H/T
-    const double arg = -Y * Gstd / RRT;
-    v.dlnKc = (arg < 600 ? HT * Y / RRT : 0.0) - (nmSmall ? 0.0 : nm / T);
-    v.ddK = (Y / RRT) * (poly_Cp(a, T) / T - 2.0 * HT / T) + (nmSmall ? 0.0 : nm / (T * T));
+    KcTerms t;
+    reaction_Kc_dKc(m, r, T, v.Kc, v.dK, &t);
+    const double HT = t.S + t.Gstd / T; // H/T
+    v.dlnKc = (t.arg < 600 ? HT * t.Y / t.RRT : 0.0) - (t.nmSmall ? 0.0 : t.nm / T);
+    v.ddK = (t.Y / t.RRT) * (poly_Cp(t.a, T) / T - 2.0 * HT / T) +
+            (t.nmSmall ? 0.0 : t.nm / (T * T));
     return v;
 }
 

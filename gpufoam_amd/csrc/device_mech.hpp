@@ -218,10 +218,19 @@ GFC_HD __forceinline__ double reaction_Kc(const DeviceMech& m, int r,
     return K * pow(kPstd / (kRR * T), nm);
 }
 
+// Intermediate terms of reaction_Kc_dKc, for callers that also need the
+// temperature derivatives of Kc.
+struct KcTerms {
+    const double* a; // the reaction's NASA-7 coefficients at T
+    double Y, nm, RRT, Gstd, S, arg;
+    bool nmSmall;
+};
+
 // KcAnddKcTbyKc (gpuThermo.H:139-169)
 GFC_HD __forceinline__ void reaction_Kc_dKc(const DeviceMech& m, int r,
                                                 double T, double& Kc,
-                                                double& dKcdTbyKc) {
+                                                double& dKcdTbyKc,
+                                                KcTerms* terms = nullptr) {
     const double Y = m.rY[r], W = m.rW[r];
     const double* a = coeffs_of(m.rhigh + 7 * r, m.rlow + 7 * r,
                                 m.rTcommon[r], T);
@@ -237,6 +246,7 @@ GFC_HD __forceinline__ void reaction_Kc_dKc(const DeviceMech& m, int r,
     double d = (S + Gstd / T) * Y / RRT;
     if (!nm_small) d = d - nm / T;
     dKcdTbyKc = d;
+    if (terms) *terms = KcTerms{a, Y, nm, RRT, Gstd, S, arg, nm_small};
 }
 
 } // namespace gfc

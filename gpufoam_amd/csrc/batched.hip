@@ -173,8 +173,9 @@ constexpr int kTeam16MinCells = 16384;
 
 // Four cells per wavefront when asked for, and by default for small
 // mechanisms once the batch fills the chip four times over: on MI355X the
-// batched solve takes 0.39x the time of the one-cell form at 65536 H2 cells
-// and 1.09x at 4096, where 1024 wavefronts leave most SIMDs idle.
+// batched solve takes 0.39x the time of the one-cell form at 65536 H2 cells,
+// 0.41x to 0.90x at 16384 and 1.13x at 4096, where 1024 wavefronts leave
+// most SIMDs idle.
 bool team16_selected(const gfc_evaluator* ev, int nCells, int& N) {
     const bool want = ev->cellsPerWave == 4 ||
                       (ev->cellsPerWave == 0 && nCells >= kTeam16MinCells);

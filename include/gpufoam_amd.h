@@ -235,8 +235,14 @@ int gfc_evaluator_set_rate_parameters(gfc_evaluator* ev, const double* params);
  *
  * gfc_adjoint_forward: gfc_solve's per-cell solve (device pointers, no RR,
  * no deltaTMin, deltaTChem not clamped) that records the accepted steps.
- * The workspace (step sizes, checkpoints every K accepted steps, per-cell
- * scratch) is allocated by this call when it grows.
+ * For nSpecie+1 <= 16 it takes the batched form's choice of four cells per
+ * wavefront (gfc_evaluator_set_batched_shape) and then returns the batched
+ * solve's results; otherwise it returns the plain solve's.  The workspace
+ * (step sizes, checkpoints every K accepted steps, per-cell scratch) is
+ * allocated by this call when it grows.
+ *
+ * gfc_adjoint_backward uses the form its forward recorded with.  The
+ * gradients of the two forms differ only by rounding.
  *
  * gfc_adjoint_backward: for the last recorded forward,
  *   Ybar     [(nS+1)*ldb]  dL/d(final Y_0..Y_nS-1, T)
@@ -247,7 +253,9 @@ int gfc_evaluator_set_rate_parameters(gfc_evaluator* ev, const double* params);
  *
  * gfc_adjoint_replay: re-integrates the recorded accepted steps from Y0
  * (same cells, same layout as the forward) with the current parameters:
- * the map whose derivative gfc_adjoint_backward returns.
+ * the map whose derivative gfc_adjoint_backward returns (one lane per
+ * cell; after a four-cells-per-wavefront forward it reproduces that
+ * forward's states to rounding).
  *
  * All three are asynchronous on `stream`. */
 int gfc_adjoint_set_checkpoint_interval(gfc_evaluator* ev, int K); /* 0: auto */

@@ -62,15 +62,19 @@ def cpu_baseline(config: str, sample: int, threads: int) -> dict:
                       f"({el:.1f} s wall on {threads} threads)"}
 
 
-def adjoint_cost(n_cells: int) -> dict:
-    """Adjoint/forward cost on the differentiation workload (H2, deltaT
-    1e-4): recording forward + reverse sweep versus the plain and the
-    batched forward solve of the same cells; loss sum(T + 1e3 Y_OH)."""
+def adjoint_cost(n_cells: int, config: str = "h2_adjoint", deltaT: float = 0.0) -> dict:
+    """Adjoint/forward cost on a workload (by default the differentiation
+    workload, H2 with deltaT 1e-4): recording forward + reverse sweep versus
+    the plain and the batched forward solve of the same cells; loss
+    sum(T + 1e3 Y_OH), or sum(T) for mechanisms without OH.  deltaT > 0
+    replaces the workload's step."""
     import torch
     from gpufoam_amd import ode_inputs
     from gpufoam_amd.evaluator import Evaluator
     from gpufoam_amd.workloads import make_states
-    m, Y0, dt0, c = make_states("h2_adjoint", n_cells)
+    m, Y0, dt0, c = make_states(config, n_cells)
+    if deltaT > 0:
+        c = dict(c, deltaT=deltaT)
     ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
     ev = Evaluator(m, ode, 0.0, max_cells=n_cells)
     dY0 = torch.tensor(Y0, device="cuda:0")
@@ -79,7 +83,9 @@ def adjoint_cost(n_cells: int) -> dict:
     n = m.n_species + 1
     W = torch.zeros((n, n_cells), dtype=torch.float64, device="cuda:0")
     W[m.n_species] = 1.0
-    W[m.index("OH")] = 1e3
+    has_oh = "OH" in m.species
+    if has_oh:
+        W[m.index("OH")] = 1e3
     g = torch.empty_like(W)
     pb = torch.empty((6 * m.n_reactions, n_cells), dtype=torch.float64, device="cuda:0")
 
@@ -103,8 +109,9 @@ def adjoint_cost(n_cells: int) -> dict:
     progress(f"adjoint workload: recording forward {t_fwd:.2f} s")
     t_bwd = timed(lambda: ev.adjoint_backward(W, g, pb))
     progress(f"adjoint workload: reverse sweep {t_bwd:.2f} s")
-    return {"workload": "h2_adjoint", "cells": n_cells, "deltaT": c["deltaT"],
-            "loss": "sum(T + 1e3*Y_OH)", "params": 6 * m.n_reactions,
+    return {"workload": config, "cells": n_cells, "deltaT": c["deltaT"],
+            "loss": "sum(T + 1e3*Y_OH)" if has_oh else "sum(T)",
+            "params": 6 * m.n_reactions,
             "forward_plain_s": t_plain, "forward_batched_s": t_batched,
             "adjoint_forward_s": t_fwd, "adjoint_reverse_s": t_bwd,
             "checkpoint_interval": ev.checkpoint_interval,
@@ -127,6 +134,10 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--adjoint-cells", type=int, default=65536,
                     help="cells of the differentiation workload for the adjoint cost (0: skip)")
+    ap.add_argument("--adjoint-config", default="h2_adjoint",
+                    help="workload of the adjoint cost (default: the differentiation workload)")
+    ap.add_argument("--adjoint-dt", type=float, default=0.0,
+                    help="CFD step of the adjoint cost in place of the workload's (0: keep)")
     ap.add_argument("--autotune", type=int, default=0, metavar="CELLS",
                     help="run the launch-shape autotuner on CELLS cells of the workload "
                          "first and time the batched solve with the shape it picks")
@@ -234,8 +245,8 @@ def main():
 
     adjoint = None
     if args.adjoint_cells:
-        progress(f"adjoint cost on {args.adjoint_cells} cells of h2_adjoint")
-        adjoint = adjoint_cost(args.adjoint_cells)
+        progress(f"adjoint cost on {args.adjoint_cells} cells of {args.adjoint_config}")
+        adjoint = adjoint_cost(args.adjoint_cells, args.adjoint_config, args.adjoint_dt)
 
     traffic = None
     tfile = os.path.join(HERE, "profiles", f"hbm_traffic_{args.config}.json")

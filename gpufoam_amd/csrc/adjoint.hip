@@ -257,11 +257,9 @@ static t16::T16Record record_of(const AdjState* st, const AdjLayout& W) {
 // Persistent grid of a four-cells-per-wavefront kernel: the wavefronts that
 // stay resident, at most one per four cells.
 template <class K>
-static int team16_grid(AdjState* st, K kernel, int nCells) {
+static int team16_grid(AdjState* st, int device, K kernel, int nCells) {
     if (st->numCUs == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&st->numCUs, hipDeviceAttributeMultiprocessorCount, dev) !=
+        if (hipDeviceGetAttribute(&st->numCUs, hipDeviceAttributeMultiprocessorCount, device) !=
                 hipSuccess ||
             st->numCUs <= 0)
             st->numCUs = 1;
@@ -378,7 +376,7 @@ int gfc_adjoint_forward(gfc_evaluator* ev, double deltaT, int nCells, int ldY,
         const t16::T16Record R = record_of(st, layout(ev));
         auto k = (N == 11) ? team16_record_kernel<11> : team16_record_kernel<16>;
         hipLaunchKernelGGL(adj_reset_queue_kernel, dim3(1), dim3(1), 0, s, st->queue);
-        hipLaunchKernelGGL(k, dim3(team16_grid(st, k, nCells)), dim3(64), 0, s, At, st->queue,
+        hipLaunchKernelGGL(k, dim3(team16_grid(st, ev->device, k, nCells)), dim3(64), 0, s, At, st->queue,
                            R);
     } else {
         const int blocks = (nCells + 63) / 64;
@@ -415,7 +413,7 @@ int gfc_adjoint_backward(gfc_evaluator* ev, const double* Ybar, int ldb, double*
         // the segment buffers sit after the forward and adjoint scratch
         double* seg = st->scratch + (W.laneD + W.adjD) * lane::kSlotStride;
         hipLaunchKernelGGL(adj_reset_queue_kernel, dim3(1), dim3(1), 0, s, st->queue);
-        hipLaunchKernelGGL(k, dim3(team16_grid(st, k, st->nCells)), dim3(64), 0, s, At,
+        hipLaunchKernelGGL(k, dim3(team16_grid(st, ev->device, k, st->nCells)), dim3(64), 0, s, At,
                            record_of(st, W), seg, W.perScratch, Ybar, (long)ldb, Y0bar,
                            paramBar, st->queue);
     } else {

@@ -81,14 +81,12 @@ def test_recording_solve_is_the_four_cell_solve(config, ncells):
     assert np.array_equal(st.cpu().numpy(), stb)
 
 
-@pytest.mark.parametrize("config,ncells", [("h2_small", 256), ("h2_adjoint", 64)])
-def test_team_gradients_equal_lane_gradients(config, ncells):
-    m, Y0, dt0, c, ode = _case(config, ncells)
+def _team_against_lane(m, Y0, dt0, c, ode, ncells, label):
     n = m.n_species + 1
     W = np.random.default_rng(0).standard_normal((n, ncells))
     res = {}
     for cpw in (1, 4):
-        print(f"case {config} n={ncells} adjoint, {cpw} cells per wavefront", flush=True)
+        print(f"case {label} n={ncells} adjoint, {cpw} cells per wavefront", flush=True)
         ev = _evaluator(m, ode, ncells, cpw)
         _, _, _, st = _forward(ev, c, Y0, dt0)
         res[cpw] = _backward(ev, m, W) + (st.cpu().numpy()[2],)
@@ -102,6 +100,21 @@ def test_team_gradients_equal_lane_gradients(config, ncells):
     print(f"compared {same.sum()} cells: state gradients {ey.max():.2e}, "
           f"rate-parameter gradients {ep.max():.2e} of the cell's largest entry")
     assert ey.max() <= 1e-7 and ep.max() <= 1e-7
+
+
+@pytest.mark.parametrize("config,ncells", [("h2_small", 256), ("h2_adjoint", 64)])
+def test_team_gradients_equal_lane_gradients(config, ncells):
+    m, Y0, dt0, c, ode = _case(config, ncells)
+    _team_against_lane(m, Y0, dt0, c, ode, ncells, config)
+
+
+@pytest.mark.parametrize("solver", ["Rosenbrock12", "Rosenbrock23"])
+def test_team_gradients_other_solvers(solver):
+    """The two- and three-stage step reverses (h2_small states, deltaT 1e-5,
+    relTol 1e-5)."""
+    m, Y0, dt0, c = make_states("h2_small", 64)
+    ode = ode_inputs(solver, relTol=1e-5, absTol=1e-10)
+    _team_against_lane(m, Y0, dt0, dict(c, deltaT=1e-5), ode, 64, f"h2_small {solver}")
 
 
 @pytest.mark.parametrize("config,ncells,maxSteps", [("h2_small", 48, None),

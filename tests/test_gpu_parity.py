@@ -44,10 +44,12 @@ def _oracle_run(cfg, n, **kw):
 
 
 def _device_run(m, Y0, dt0, c, ode, rho, batched, graph=False, deltaTChemMax=1e300,
-                Treact=0.0):
+                Treact=0.0, cells_per_wave=None):
     from gpufoam_amd.evaluator import Evaluator
     n = Y0.shape[1]
     ev = Evaluator(m, ode, Treact, max_cells=n)
+    if cells_per_wave is not None:
+        ev.set_batched_shape(cells_per_wave)
     dY0 = torch.tensor(Y0, device="cuda")
     dYn = torch.empty_like(dY0)
     dRR = torch.zeros((m.n_species, n), dtype=torch.float64, device="cuda")
@@ -122,6 +124,21 @@ def test_other_solvers(solver, batched):
     ref = Oracle(m).solve_cells(ode, Y0, 1e-5, dt0, rho=rho, want_rr=True)
     got = _device_run(m, Y0, dt0, c | {"deltaT": 1e-5}, ode, rho, batched=batched)
     _check(m, got, ref, tight_dt=not batched)
+
+
+@pytest.mark.parametrize("solver", ["Rosenbrock12", "Rosenbrock23"])
+def test_four_cells_per_wavefront_other_solvers(solver):
+    """The two- and three-stage tableaux through the 16-lane-team form
+    (set_batched_shape(4)), as test_other_solvers."""
+    from oracle_lib import Oracle
+    m, Y0, dt0, c = make_states("h2_small", 128)
+    ode = ode_inputs(solver, relTol=1e-5, absTol=1e-10)
+    rho = np.ones(128)
+    ref = Oracle(m).solve_cells(ode, Y0, 1e-5, dt0, rho=rho, want_rr=True)
+    print(f"case h2_small n=128 {solver} cells per wavefront 4", flush=True)
+    got = _device_run(m, Y0, dt0, c | {"deltaT": 1e-5}, ode, rho, batched=True,
+                      cells_per_wave=4)
+    _check(m, got, ref, tight_dt=False)
 
 
 @pytest.mark.parametrize("cfg,n", [("h2_small", 384), ("h2_adjoint", 96)])

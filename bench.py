@@ -9,13 +9,20 @@ solve of all cells (the batched persistent kernel replayed as a hipGraph),
 with the chemical time step reset to its initial value before each step so
 every step does the same work.  Inputs are resident in HBM.
 
-Printed (one JSON line): the headline metric and, beside it, the plain
-per-cell kernel (one lane per cell, the reference's mapping) on the same
-cells, the FP64 roofline fraction of the batched kernel, the CPU baseline
-(gpuFoam's own per-cell code built for the host, or the oracle restatement
-where that build is absent, on the host cores over a bounded sample of the
-same states) and the adjoint/forward cost on the differentiation workload
-(10-species H2, 65536 cells, deltaT 1e-4).
+Printed (one JSON line): the headline metric and, beside it,
+  * the drop-in launch (the same per-cell kernel over one grid of cells,
+    no work stealing) and the one-lane-per-cell kernel (the reference's
+    mapping) on the same cells;
+  * the batching workload (GRI, 1048576 cells, 10% igniting, deltaT 1e-3):
+    persistent + hipGraph against the drop-in launch, with its attempt
+    histogram;
+  * the FP64 roofline fraction of the batched kernel;
+  * the CPU baseline (gpuFoam's own per-cell code built for the host, or
+    the oracle restatement where that build is absent) on the host cores
+    over a bounded sample of the same cells, and the error of the GPU
+    result against that CPU result on the sample;
+  * the adjoint/forward cost on the differentiation workload (10-species
+    H2, 65536 cells, deltaT 1e-4).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--config gri_production]
 """
@@ -40,26 +47,37 @@ def progress(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(config: str, sample: int, threads: int) -> dict:
-    """gpuFoam's per-cell solver built for the host (oracle/_ref) or the
-    oracle, timed on `sample` cells of the same state law."""
+def cpu_baseline(m, Y0, dt0, c, config: str, sample: int, threads: int,
+                 gpu_Ynew: np.ndarray) -> dict:
+    """gpuFoam's per-cell solver built for the host (oracle/_ref, shipped
+    with the tree) or, where that build is absent, the oracle, timed on the
+    first `sample` cells of the benchmarked workload.  The CPU result on
+    the sample is also the yardstick of the GPU result's error."""
     sys.path.insert(0, os.path.join(HERE, "tests"))
     from gpufoam_amd import ode_inputs
-    from gpufoam_amd.workloads import make_states
     import oracle_lib
-    m, Y0, dt0, c = make_states(config, sample)
+    Ys = np.ascontiguousarray(Y0[:, :sample])
+    dts = np.ascontiguousarray(dt0[:sample])
     ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
     if oracle_lib.ref_available():
         runner, kind = oracle_lib.Reference(m), "reference"
     else:
         runner, kind = oracle_lib.Oracle(m), "port"
-    runner.solve_cells(ode, Y0[:, :64], c["deltaT"], dt0[:64], threads=threads)  # warm
+        progress("cpu baseline: oracle/_ref/libgfref.so absent, timing the oracle (kind port)")
+    runner.solve_cells(ode, Ys[:, :64], c["deltaT"], dts[:64], threads=threads)  # warm
     t = time.perf_counter()
-    runner.solve_cells(ode, Y0, c["deltaT"], dt0, threads=threads)
+    ref = runner.solve_cells(ode, Ys, c["deltaT"], dts, threads=threads)
     el = time.perf_counter() - t
+    nS = m.n_species
+    Yc, Yg = ref[0], gpu_Ynew[:, :sample]
+    errY = np.abs(Yg[:nS] - Yc[:nS]) / (1e-6 * np.abs(Yc[:nS]) + 1e-10)
+    errT = np.abs(Yg[nS] - Yc[nS]) / Yc[nS]
     return {"value": sample / el, "unit": "cells/s", "cores": threads, "kind": kind,
-            "sample": f"{sample} cells of the same {config} state law, one CFD step "
-                      f"({el:.1f} s wall on {threads} threads)"}
+            "sample": f"first {sample} cells of the benchmarked {config} cells, one CFD "
+                      f"step ({el:.1f} s wall on {threads} threads)",
+            "max_err_over_tol": float(errY.max()), "max_T_rel": float(errT.max()),
+            "error_definition": "GPU batched result vs this CPU result on the sample: "
+                                "max |dY|/(1e-6|Y|+1e-10) over species, max |dT|/T"}
 
 
 def adjoint_cost(n_cells: int, config: str = "h2_adjoint", deltaT: float = 0.0) -> dict:
@@ -99,8 +117,10 @@ def adjoint_cost(n_cells: int, config: str = "h2_adjoint", deltaT: float = 0.0) 
         return e0.elapsed_time(e1) / 1e3
 
     ddt = ddt0.clone()
+    ev.solve(c["deltaT"], dY0[:, :64], ddt[:64], Ynew=dYn[:, :64], ldY=n_cells)  # warm
+    ddt.copy_(ddt0)
     t_plain = timed(lambda: ev.solve(c["deltaT"], dY0, ddt, Ynew=dYn))
-    progress(f"adjoint workload: plain forward {t_plain:.2f} s")
+    progress(f"adjoint workload: drop-in forward {t_plain:.2f} s")
     ddt.copy_(ddt0)
     t_batched = timed(lambda: ev.solve(c["deltaT"], dY0, ddt, Ynew=dYn, batched=True))
     progress(f"adjoint workload: batched forward {t_batched:.2f} s")
@@ -115,8 +135,56 @@ def adjoint_cost(n_cells: int, config: str = "h2_adjoint", deltaT: float = 0.0) 
             "forward_plain_s": t_plain, "forward_batched_s": t_batched,
             "adjoint_forward_s": t_fwd, "adjoint_reverse_s": t_bwd,
             "checkpoint_interval": ev.checkpoint_interval,
+            "adjoint_over_batched_forward": (t_fwd + t_bwd) / t_batched,
             "adjoint_over_plain_forward": (t_fwd + t_bwd) / t_plain,
-            "adjoint_over_batched_forward": (t_fwd + t_bwd) / t_batched}
+            "target": "adjoint_over_batched_forward <= 4"}
+
+
+def batching_block(n_cells: int, graph: bool) -> dict:
+    """The batching workload: persistent work-stealing launch (replayed as a
+    hipGraph) against the drop-in launch of the same kernel, one CFD step
+    each on the same cells, and the per-cell attempt histogram."""
+    import torch
+    from gpufoam_amd import ode_inputs
+    from gpufoam_amd.evaluator import Evaluator
+    from gpufoam_amd.workloads import make_states
+    m, Y0, dt0, c = make_states("gri_batching", n_cells)
+    ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
+    ev = Evaluator(m, ode, 0.0, max_cells=n_cells)
+    dY0 = torch.tensor(Y0, device="cuda:0")
+    del Y0
+    dYn = torch.empty_like(dY0)
+    ddt0 = torch.tensor(dt0, device="cuda:0")
+    ddt = ddt0.clone()
+    st = torch.zeros((3, n_cells), dtype=torch.int32, device="cuda:0")
+    ev.solve(c["deltaT"], dY0[:, :256], ddt[:256], Ynew=dYn[:, :256], ldY=n_cells)  # warm
+    ev.solve(c["deltaT"], dY0[:, :256], ddt[:256], Ynew=dYn[:, :256], ldY=n_cells,
+             batched=True)
+    torch.cuda.synchronize()
+
+    def timed(batched):
+        ddt.copy_(ddt0)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ev.solve(c["deltaT"], dY0, ddt, Ynew=dYn, stats=st, batched=batched,
+                 graph=graph and batched)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3
+
+    t_b = timed(True)
+    progress(f"batching workload: persistent {t_b:.2f} s")
+    att = st[1].cpu().numpy()
+    t_p = timed(False)
+    progress(f"batching workload: drop-in {t_p:.2f} s")
+    q = np.percentile(att, [50, 90, 99])
+    return {"workload": "gri_batching", "cells": n_cells, "deltaT": c["deltaT"],
+            "batched_s": t_b, "plain_s": t_p,
+            "batched_cells_per_s": n_cells / t_b, "plain_cells_per_s": n_cells / t_p,
+            "batched_over_plain": t_p / t_b,
+            "attempts": {"mean": float(att.mean()), "p50": float(q[0]), "p90": float(q[1]),
+                         "p99": float(q[2]), "max": int(att.max())}}
 
 
 def main():
@@ -127,8 +195,12 @@ def main():
     ap.add_argument("--config", default="gri_production")
     ap.add_argument("--cells", type=int, default=0, help="override the cell count")
     ap.add_argument("--plain-sample", type=int, default=0,
-                    help="cells for the plain-kernel line (0: the whole workload)")
+                    help="cells for the drop-in and lane-kernel lines (0: the whole workload)")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-lane", action="store_true",
+                    help="skip the one-lane-per-cell line")
+    ap.add_argument("--batching-cells", type=int, default=1048576,
+                    help="cells of the batching workload block (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
@@ -220,33 +292,54 @@ def main():
     flops = work_flops(m, st[1], st[2], stages)
     achieved = flops / (ms_per_step / 1e3) / 1e12
 
-    # plain per-cell kernel (one lane per cell) on the same states, one step
-    plain_value, ns = None, 0
-    if not args.no_plain:
+    Yg = dYn.cpu().numpy()
+
+    # the drop-in launch and the one-lane-per-cell kernel on the same states
+    def plain_line(lane: bool):
         ns = min(args.plain_sample or n, n)
         dY0p = dY0[:, :ns].contiguous()
         dYnp = torch.empty_like(dY0p)
         ddtp = ddt0[:ns].clone()
-        ev.solve(c["deltaT"], dY0p[:, :64], ddtp[:64], Ynew=dYnp[:, :64], ldY=ns)  # warm
+        ev.solve(c["deltaT"], dY0p[:, :64], ddtp[:64], Ynew=dYnp[:, :64], ldY=ns,
+                 lane=lane)  # warm
         torch.cuda.synchronize()
         ddtp.copy_(ddt0[:ns])
         p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         p0.record(stream)
-        ev.solve(c["deltaT"], dY0p, ddtp, Ynew=dYnp, ldY=ns)
+        ev.solve(c["deltaT"], dY0p, ddtp, Ynew=dYnp, ldY=ns, lane=lane)
         p1.record(stream)
         torch.cuda.synchronize()
-        plain_value = ns / (p0.elapsed_time(p1) / 1e3)
-        progress(f"plain: {plain_value:.0f} cells/s")
+        return ns / (p0.elapsed_time(p1) / 1e3), ns
+
+    plain_value = lane_value = None
+    ns = nl = 0
+    if not args.no_plain:
+        plain_value, ns = plain_line(False)
+        progress(f"drop-in: {plain_value:.0f} cells/s")
+        if not args.no_lane:
+            lane_value, nl = plain_line(True)
+            progress(f"one lane per cell: {lane_value:.0f} cells/s")
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    cpu = None if args.no_cpu else cpu_baseline(args.config, min(args.cpu_sample, n), threads)
-    if cpu:
-        progress(f"cpu baseline: {cpu['value']:.0f} cells/s on {threads} threads")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(m, Y0, dt0, c, args.config, min(args.cpu_sample, n), threads, Yg)
+        progress(f"cpu baseline: {cpu['value']:.0f} cells/s on {threads} threads; "
+                 f"GPU error {cpu['max_err_over_tol']:.3g} of tol, T {cpu['max_T_rel']:.3g}")
+    del Yg
 
     adjoint = None
     if args.adjoint_cells:
         progress(f"adjoint cost on {args.adjoint_cells} cells of {args.adjoint_config}")
         adjoint = adjoint_cost(args.adjoint_cells, args.adjoint_config, args.adjoint_dt)
+
+    batching = None
+    if args.batching_cells:
+        del dY0, dYn
+        ev = None
+        torch.cuda.empty_cache()
+        progress(f"batching workload: {args.batching_cells} cells")
+        batching = batching_block(args.batching_cells, graph)
 
     traffic = None
     tfile = os.path.join(HERE, "profiles", f"hbm_traffic_{args.config}.json")
@@ -271,14 +364,25 @@ def main():
                    "species": m.n_species, "reactions": m.n_reactions, "cells": n,
                    "solver": c["solver"], "relTol": c["relTol"], "absTol": c["absTol"],
                    "deltaT": c["deltaT"], "graph": graph},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": traffic,
-                     "work_model_flops_per_attempt": attempt_flops(m, stages)["attempt"]},
+                     "work_model_flops_per_attempt": attempt_flops(m, stages)["attempt"],
+                     "work_model": "operation counts of the CPU restatement: +,-,*,/ and "
+                                   "each exp/log/pow = 1 flop; one Jacobian, LU and "
+                                   f"{stages} solves per attempt, one extra RHS per accepted "
+                                   "step, from the kernel's own attempt/accept counters; "
+                                   "peak = MI355X FP64 vector (no MFMA on this path)"},
         "cpu_baseline": cpu,
+        "max_err_over_tol": cpu["max_err_over_tol"] if cpu else None,
+        "max_T_rel": cpu["max_T_rel"] if cpu else None,
         "plain_cells_per_s": plain_value,
         "plain_sample_cells": ns,
         "batched_over_plain": value / plain_value if plain_value else None,
+        "lane_cells_per_s": lane_value,
+        "lane_sample_cells": nl,
+        "batched_over_lane": value / lane_value if lane_value else None,
+        "batching": batching,
         "adjoint": adjoint,
         "autotune": tuned,
         "attempts_per_cell": float(st[1].mean()),

@@ -33,6 +33,7 @@ CELL_SKIPPED = 0x4
 HOST_POINTERS = 0x1
 BATCHED = 0x2
 GRAPH = 0x4
+LANE_PER_CELL = 0x8
 
 
 class GfcError(RuntimeError):
@@ -110,6 +111,13 @@ def lib() -> C.CDLL:
                                               vp, vp]),
             "gfc_adjoint_backward": (C.c_int, [vp, vp, C.c_int, vp, vp, vp]),
             "gfc_adjoint_replay": (C.c_int, [vp, vp, C.c_int, vp, vp]),
+            "gfc_calculate": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp, C.c_uint]),
+            "gfc_tc": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp, C.c_uint]),
+            "gfc_reaction_rr": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp,
+                                          C.c_uint]),
+            "gfc_specie_reaction_rr": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp,
+                                                 vp, C.c_uint]),
+            "gfc_qdot": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, C.c_uint]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name, None)
@@ -134,6 +142,7 @@ def exported_symbols() -> list[str]:
         "gfc_evaluator_get_rate_parameters", "gfc_evaluator_set_rate_parameters",
         "gfc_adjoint_set_checkpoint_interval", "gfc_adjoint_checkpoint_interval",
         "gfc_adjoint_forward", "gfc_adjoint_backward", "gfc_adjoint_replay",
+        "gfc_calculate", "gfc_tc", "gfc_reaction_rr", "gfc_specie_reaction_rr", "gfc_qdot",
     ]
 
 

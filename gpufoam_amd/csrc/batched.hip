@@ -54,6 +54,7 @@ using KernelFn = void (*)(SolveArgs, int*);
 using BlockFn = void (*)(SolveArgs, int*, double*, long);
 
 constexpr int kBlockThreads = 512;
+constexpr int kLanesPerWave = 64;
 
 BlockFn block_kernel_for(int NA) {
     if (NA == 8) return block_solve_kernel<8, 4>;
@@ -182,48 +183,82 @@ bool team16_selected(const gfc_evaluator* ev, int nCells, int& N) {
     return want && team16_kernel_for(ev->dmTeam, N) != nullptr;
 }
 
-int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
-                   bool graph) {
+// The kernel and persistent grid of a launch of nCells cells: four cells
+// per wavefront, a wavefront per cell, a workgroup per cell, or (pl.team
+// and pl.blk both null) one lane per cell.
+static void choose_plan(gfc_evaluator* ev, int nCells, Plan& pl, int& cellsPerBlock) {
     auto* b = static_cast<BatchedState*>(ev->batched);
-    const int n = A.m.nS + 1;
-    if (ev->cellsPerWave == 64) {
-        // one lane per cell
-        const int blocks = (A.nCells + 63) / 64;
-        hipLaunchKernelGGL(lane_solve_kernel, dim3(blocks), dim3(64), 0, s, A,
-                           ev->laneScratch, ev->lanePer);
-        return hipFail(hipGetLastError(), "lane launch") ? GFC_ERR_HIP : GFC_OK;
-    }
-    int N = 0;
-    Plan pl;
-    KernelFn t16 = team16_selected(ev, A.nCells, N) ? team16_kernel_for(ev->dmTeam, N)
+    const int n = ev->nS + 1;
+    pl = Plan{};
+    cellsPerBlock = 1;
+    if (ev->cellsPerWave == 64) return;
+    int N16 = 0, N = 0;
+    KernelFn t16 = team16_selected(ev, nCells, N16) ? team16_kernel_for(ev->dmTeam, N16)
                                                     : nullptr;
-    pl.team = t16 ? t16 : team_kernel_for(n, N);
+    KernelFn team = team_kernel_for(n, N);
     if (t16) {
         int perCU = resident_blocks(t16);
         if (ev->wavesPerBlock > 0 && ev->wavesPerBlock < perCU) perCU = ev->wavesPerBlock;
+        pl.team = t16;
         pl.grid = b->numCUs * perCU;
-        pl.key = 2000 + N;
-    } else if (pl.team && ev->dmTeam.teamWidth <= kTeamWidth) {
-        int perCU = resident_blocks(pl.team);
+        pl.key = 2000 + N16;
+        cellsPerBlock = kLanesPerWave / 16;
+    } else if (team && ev->dmTeam.teamWidth <= kTeamWidth) {
+        int perCU = resident_blocks(team);
         if (ev->wavesPerBlock > 0 && ev->wavesPerBlock < perCU) perCU = ev->wavesPerBlock;
+        pl.team = team;
         pl.grid = b->numCUs * perCU;
         pl.key = N;
     } else if (b->blkNA > 0) {
-        pl.team = nullptr;
         pl.blk = block_kernel_for(b->blkNA);
         pl.grid = b->blkGrid;
         pl.key = 1000 + b->blkNA;
-    } else {
-        // mechanism outside both kernels' capacity: one lane per cell
-        const int blocks = (A.nCells + 63) / 64;
-        hipLaunchKernelGGL(lane_solve_kernel, dim3(blocks), dim3(64), 0, s, A,
-                           ev->laneScratch, ev->lanePer);
-        return hipFail(hipGetLastError(), "lane launch") ? GFC_ERR_HIP : GFC_OK;
     }
+}
+
+static int lane_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s) {
+    const int blocks = (A.nCells + 63) / 64;
+    hipLaunchKernelGGL(lane_solve_kernel, dim3(blocks), dim3(64), 0, s, A,
+                       ev->laneScratch, ev->lanePer);
+    return hipFail(hipGetLastError(), "lane launch") ? GFC_ERR_HIP : GFC_OK;
+}
+
+int plain_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s, bool lane) {
+    Plan pl;
+    int perBlock = 1;
+    if (!lane) choose_plan(ev, A.nCells, pl, perBlock);
+    if (lane || (!pl.team && !pl.blk)) return lane_launch(ev, A, s);
+    auto* b = static_cast<BatchedState*>(ev->batched);
+    SolveArgs At = A;
+    At.m = ev->dmTeam;
+    if (pl.blk) {
+        // one workgroup per cell: each resident workgroup owns a scratch
+        // slot, so the grid stays at the resident count and deals the cells
+        // round-robin
+        const int grid = pl.grid < A.nCells ? pl.grid : A.nCells;
+        hipLaunchKernelGGL(pl.blk, dim3(grid), dim3(kBlockThreads), 0, s, At, nullptr,
+                           b->blkScratch, b->blkPerWG);
+    } else {
+        // one wavefront per cell (or per four cells): the grid covers the
+        // cells and the hardware dispatcher fills the CUs
+        const int grid = (A.nCells + perBlock - 1) / perBlock;
+        hipLaunchKernelGGL(pl.team, dim3(grid), dim3(64), 0, s, At, nullptr);
+    }
+    return hipFail(hipGetLastError(), "drop-in launch") ? GFC_ERR_HIP : GFC_OK;
+}
+
+int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
+                   bool graph) {
+    auto* b = static_cast<BatchedState*>(ev->batched);
+    Plan pl;
+    int perBlock = 1;
+    choose_plan(ev, A.nCells, pl, perBlock);
+    if (!pl.team && !pl.blk) return lane_launch(ev, A, s);
+    const bool t16 = perBlock > 1;
     SolveArgs At = A;
     At.m = ev->dmTeam;
     if (t16) {
-        const int waves = (A.nCells + 3) / 4;
+        const int waves = (A.nCells + perBlock - 1) / perBlock;
         if (pl.grid > waves) pl.grid = waves > 0 ? waves : 1;
     } else if (pl.grid > A.nCells) {
         pl.grid = A.nCells > 0 ? A.nCells : 1;

@@ -877,11 +877,20 @@ block_solve_kernel(SolveArgs A, int* queue, double* scratch, long perWG) {
     double* Jg = scratch + (long)blockIdx.x * perWG;
     double* LUg = Jg + (long)NA * NC * kBT;
     int tog = 0;
+    // without a queue (the drop-in launch) the cells are dealt round-robin:
+    // block b integrates cells b, b + gridDim.x, ...
+    int next = blockIdx.x;
 
     for (;;) {
-        if (t == 0) L.cell = atomicAdd(queue, 1);
-        __syncthreads();
-        const int cell = __builtin_amdgcn_readfirstlane(L.cell);
+        int cell;
+        if (queue) {
+            if (t == 0) L.cell = atomicAdd(queue, 1);
+            __syncthreads();
+            cell = __builtin_amdgcn_readfirstlane(L.cell);
+        } else {
+            cell = next;
+            next += gridDim.x;
+        }
         if (cell >= A.nCells) break;
         if (t < n) L.y[t] = A.Y0[t * ld + cell];
         const double p = A.Y0[(long)(nS + 1) * ld + cell];

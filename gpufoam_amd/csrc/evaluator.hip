@@ -514,9 +514,8 @@ int gfc_solve(gfc_evaluator* ev, double deltaT, double deltaTChemMax,
         st = batched_launch(ev, A, s, (flags & GFC_GRAPH) != 0);
         if (st != GFC_OK) return st;
     } else {
-        const int blocks = (nCells + 63) / 64;
-        hipLaunchKernelGGL(lane_solve_kernel, dim3(blocks), dim3(64), 0, s, A,
-                           ev->laneScratch, ev->lanePer);
+        st = plain_launch(ev, A, s, (flags & GFC_LANE_PER_CELL) != 0);
+        if (st != GFC_OK) return st;
     }
     if (hipFail(hipGetLastError(), "kernel launch")) return GFC_ERR_HIP;
     if (deltaTMin)

@@ -51,6 +51,10 @@ int batched_init(gfc_evaluator* ev);
 void batched_free(gfc_evaluator* ev);
 int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
                    bool graph);
+// The drop-in (non-persistent) launch of the same kernels: a wavefront per
+// cell over a grid of nCells, four cells per wavefront, or the workgroup
+// kernel dealing cells round-robin; lane = true: one lane per cell.
+int plain_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s, bool lane);
 // Whether a launch of nCells cells runs four cells per wavefront (the
 // evaluator's shape, or the automatic choice); N: the kernels' size.
 bool team16_selected(const gfc_evaluator* ev, int nCells, int& N);

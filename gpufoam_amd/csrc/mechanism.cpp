@@ -568,4 +568,37 @@ int gfc_ode_inputs_from_foam(const char* text, gfc_ode_inputs* out,
     }
 }
 
+// OpenFOAM's Switch words (on/off, yes/no, true/false, y/n, t/f, none).
+static int switch_value(const std::string& w) {
+    for (const char* t : {"on", "yes", "true", "y", "t"})
+        if (w == t) return 1;
+    for (const char* f : {"off", "no", "false", "n", "f", "none"})
+        if (w == f) return 0;
+    return -1;
+}
+
+int gfc_chemistry_switches_from_foam(const char* text, int* chemistry,
+                                     double* initialChemicalTimeStep,
+                                     double* maxChemicalTimeStep) {
+    if (!text) {
+        gfc::setLastError("gfc_chemistry_switches_from_foam: null argument");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    try {
+        const gfc::Dict d = gfc::parseDict(text);
+        const int on = switch_value(d.word("chemistry"));
+        if (on < 0) throw gfc::ParseError("chemistry: expected a switch (on/off)");
+        if (chemistry) *chemistry = on;
+        if (initialChemicalTimeStep)
+            *initialChemicalTimeStep = d.scalar("initialChemicalTimeStep");
+        if (maxChemicalTimeStep)
+            *maxChemicalTimeStep = d.scalarOr("maxChemicalTimeStep", gfc::kGreat);
+        gfc::setLastError("");
+        return GFC_OK;
+    } catch (const std::exception& e) {
+        gfc::setLastError(e.what());
+        return GFC_ERR_PARSE;
+    }
+}
+
 } // extern "C"

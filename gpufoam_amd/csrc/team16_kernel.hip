@@ -30,7 +30,7 @@ using namespace t16;
 // ---------------------------------------------------------------- kernel --
 
 // The persistent solve: each team takes cells from the queue until it is
-// empty.  The kernels below run two wavefronts per SIMD: the register cap
+// empty (queue == nullptr: one cell per team, fixed by the block index).  The kernels below run two wavefronts per SIMD: the register cap
 // (256) spills ~350 B per lane, but the second wavefront hides the LDS and
 // DPP latency that bounds a lone one; on 65536 H2 cells this runs 1.7x
 // faster than at one per SIMD.
@@ -45,10 +45,20 @@ __device__ __forceinline__ void t16_run(const SolveArgs& A, int* queue, const T1
     const long ld = A.ldY;
     const RosTableau& tb = A.tab;
 
+    // without a queue (the drop-in launch, grid = cells/4) team t of
+    // block b integrates cell 4b+t once
+    bool taken = false;
     for (;;) {
-        int c = 0;
-        if (mem == 0) c = atomicAdd(queue, 1);
-        const int cell = team_bcast_i(c, lane, 0);
+        int cell;
+        if (queue) {
+            int c = 0;
+            if (mem == 0) c = atomicAdd(queue, 1);
+            cell = team_bcast_i(c, lane, 0);
+        } else {
+            if (taken) break;
+            taken = true;
+            cell = (int)blockIdx.x * (kLanes / kTW) + team;
+        }
         if (cell >= A.nCells) break;
 
         double y = (mem < n) ? A.Y0[mem * ld + cell] : 0.0;

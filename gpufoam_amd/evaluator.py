@@ -33,6 +33,28 @@ def _ptr(a):
     return a.data_ptr()
 
 
+def _check_device(name, t, rows, ld, dtype="float64", device=None, min_cols=None):
+    """Rejects a device buffer the kernels would index out of bounds: it must
+    be a contiguous torch tensor of `dtype` with `rows` rows of exactly `ld`
+    columns (at least `min_cols` when given instead), on `device`."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if str(t.dtype) != f"torch.{dtype}":
+        raise ValueError(f"{name}: dtype {t.dtype}, expected torch.{dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: must be contiguous")
+    if t.device.type != "cuda" or (device is not None and t.device != device):
+        raise ValueError(f"{name}: on {t.device}, expected {device or 'a cuda device'}")
+    if t.dim() != 2 or t.shape[0] < rows:
+        raise ValueError(f"{name}: shape {tuple(t.shape)}, expected [{rows}, {ld}]")
+    if min_cols is not None:
+        if t.shape[1] < min_cols:
+            raise ValueError(f"{name}: {t.shape[1]} columns, at least {min_cols} needed")
+    elif t.shape[1] != ld:
+        raise ValueError(f"{name}: {t.shape[1]} columns, expected exactly {ld}")
+
+
 class Evaluator:
     def __init__(self, mech: Mechanism, ode: _capi.OdeInputs, Treact: float = 0.0,
                  max_cells: int = 1 << 16):
@@ -76,11 +98,13 @@ class Evaluator:
     def solve(self, deltaT: float, Y0, dtChem, Ynew=None, RR=None, rho=None,
               deltaTMin=None, stats=None, deltaTChemMax: float = 1e300,
               batched: bool = False, graph: bool = False, stream=None,
-              ldY: int | None = None) -> None:
+              ldY: int | None = None, lane: bool = False) -> None:
         """One CFD step on device tensors (SoA layout, see gpufoam_amd.h).
 
         Y0 [nS+2, ld], Ynew like Y0 (may be Y0), RR [nS, ld], rho [n],
         dtChem [n] (in/out), deltaTMin [1], stats int32 [3, ld].
+        batched: the persistent work-stealing launch; otherwise one launch
+        over the cells (the drop-in form), with lane=True one lane per cell.
         Asynchronous on ``stream`` (default: torch's current stream)."""
         n = dtChem.shape[0]
         ld = ldY if ldY is not None else Y0.shape[1]
@@ -89,6 +113,8 @@ class Evaluator:
             flags |= _capi.BATCHED
         if graph:
             flags |= _capi.GRAPH
+        if lane:
+            flags |= _capi.LANE_PER_CELL
         if stream is None:
             import torch
             stream = torch.cuda.current_stream().cuda_stream
@@ -98,7 +124,7 @@ class Evaluator:
 
     def solve_host(self, deltaT: float, Y0: np.ndarray, dtChem: np.ndarray,
                    rho: np.ndarray | None = None, deltaTChemMax: float = 1e300,
-                   want_rr: bool = False, batched: bool = False):
+                   want_rr: bool = False, batched: bool = False, lane: bool = False):
         """Host-array solve (staged through the evaluator's device buffers).
         Returns (Ynew, dtChem_new, deltaTMin, stats, RR)."""
         Y0 = np.ascontiguousarray(Y0, dtype=np.float64)
@@ -109,7 +135,8 @@ class Evaluator:
         stats = np.zeros((3, n), dtype=np.int32)
         rr = np.zeros((self.nS, n)) if want_rr else None
         rho_a = np.ascontiguousarray(rho if rho is not None else np.ones(n), dtype=np.float64)
-        flags = _capi.HOST_POINTERS | (_capi.BATCHED if batched else 0)
+        flags = (_capi.HOST_POINTERS | (_capi.BATCHED if batched else 0) |
+                 (_capi.LANE_PER_CELL if lane else 0))
         _capi.check(_capi.lib().gfc_solve(
             self._h, deltaT, deltaTChemMax, n, n, _ptr(rho_a) if want_rr else None,
             _ptr(Y0), _ptr(Yn), _ptr(rr), _ptr(dt), _ptr(dtmin), _ptr(stats), None, flags))
@@ -144,8 +171,16 @@ class Evaluator:
 
     def adjoint_forward(self, deltaT: float, Y0, dtChem, Ynew=None, stats=None,
                         stream=None) -> None:
-        """Recording forward solve on device tensors (SoA [nS+2, ld])."""
+        """Recording forward solve on device tensors (SoA [nS+2, ld]);
+        Ynew and stats, when given, share Y0's leading dimension."""
         n = dtChem.shape[0]
+        ld = Y0.shape[1]
+        _check_device("Y0", Y0, self.nE, ld, min_cols=n)
+        _check_device("dtChem", dtChem.reshape(1, -1), 1, n, device=Y0.device)
+        if Ynew is not None:
+            _check_device("Ynew", Ynew, self.nE, ld, device=Y0.device)
+        if stats is not None:
+            _check_device("stats", stats, 3, ld, dtype="int32", device=Y0.device)
         if stream is None:
             import torch
             stream = torch.cuda.current_stream().cuda_stream
@@ -154,7 +189,15 @@ class Evaluator:
             _ptr(stats), stream))
 
     def adjoint_backward(self, Ybar, Y0bar, paramBar=None, stream=None) -> None:
-        """Ybar, Y0bar [nS+1, ld]; paramBar [6 nR, ld] or None."""
+        """Ybar, Y0bar [nS+1, ld]; paramBar [6 nR, ld] or None, all with
+        the same leading dimension ld (the kernels index Y0bar and paramBar
+        with Ybar's)."""
+        ld = Ybar.shape[1] if Ybar.dim() == 2 else -1
+        _check_device("Ybar", Ybar, self.nS + 1, ld)
+        _check_device("Y0bar", Y0bar, self.nS + 1, ld, device=Ybar.device)
+        if paramBar is not None:
+            _check_device("paramBar", paramBar, 6 * self.mech.n_reactions, ld,
+                          device=Ybar.device)
         if stream is None:
             import torch
             stream = torch.cuda.current_stream().cuda_stream
@@ -162,12 +205,70 @@ class Evaluator:
             self._h, _ptr(Ybar), Ybar.shape[1], _ptr(Y0bar), _ptr(paramBar), stream))
 
     def adjoint_replay(self, Y0, Ynew, stream=None) -> None:
-        """Re-integrates the recorded accepted steps from Y0 into Ynew."""
+        """Re-integrates the recorded accepted steps from Y0 into Ynew
+        (both [nS+2, ld] with the same ld)."""
+        ld = Y0.shape[1] if Y0.dim() == 2 else -1
+        _check_device("Y0", Y0, self.nE, ld)
+        _check_device("Ynew", Ynew, self.nE, ld, device=Y0.device)
         if stream is None:
             import torch
             stream = torch.cuda.current_stream().cuda_stream
         _capi.check(_capi.lib().gfc_adjoint_replay(
             self._h, _ptr(Y0), Y0.shape[1], _ptr(Ynew), stream))
+
+    # ------------------------------------------------------------ queries
+    # gpuChemistryModel's queries at the current state
+    # (gpuChemistryModel.C:146-466) on device tensors: Y [nS+2, ld] SoA,
+    # rho [n]; results are written into the given output tensors.
+    def _stream(self, stream):
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        return stream
+
+    def _state_checks(self, rho, Y):
+        n = rho.shape[0]
+        _check_device("Y", Y, self.nE, Y.shape[1] if Y.dim() == 2 else -1, min_cols=n)
+        _check_device("rho", rho.reshape(1, -1), 1, n, device=Y.device)
+        return n, Y.shape[1]
+
+    def calculate(self, rho, Y, RR, stream=None) -> None:
+        """RR [nS, ld] = W_i sum_r nu_ir omega_r (calculate, :146-183)."""
+        n, ld = self._state_checks(rho, Y)
+        _check_device("RR", RR, self.nS, ld, device=Y.device)
+        _capi.check(_capi.lib().gfc_calculate(self._h, n, ld, _ptr(rho), _ptr(Y), _ptr(RR),
+                                              self._stream(stream), 0))
+
+    def tc(self, rho, Y, out, stream=None) -> None:
+        """out [n] = chemical time scale (tc, :366-443)."""
+        n, ld = self._state_checks(rho, Y)
+        _check_device("tc", out.reshape(1, -1), 1, n, device=Y.device)
+        _capi.check(_capi.lib().gfc_tc(self._h, n, ld, _ptr(rho), _ptr(Y), _ptr(out),
+                                       self._stream(stream), 0))
+
+    def reactionRR(self, reaction: int, rho, Y, out, stream=None) -> None:
+        """out [n] = omega of one reaction (reactionRR, :186-240)."""
+        n, ld = self._state_checks(rho, Y)
+        _check_device("rr", out.reshape(1, -1), 1, n, device=Y.device)
+        _capi.check(_capi.lib().gfc_reaction_rr(self._h, reaction, n, ld, _ptr(rho),
+                                                _ptr(Y), _ptr(out), self._stream(stream), 0))
+
+    def specieReactionRR(self, reaction: int, rho, Y, RR, stream=None) -> None:
+        """RR [nS, ld] = W_i nu_ir omega_r of one reaction (specieReactionRR,
+        :242-317)."""
+        n, ld = self._state_checks(rho, Y)
+        _check_device("RR", RR, self.nS, ld, device=Y.device)
+        _capi.check(_capi.lib().gfc_specie_reaction_rr(self._h, reaction, n, ld, _ptr(rho),
+                                                       _ptr(Y), _ptr(RR),
+                                                       self._stream(stream), 0))
+
+    def Qdot(self, RR, out, stream=None) -> None:
+        """out [n] = -sum_i hf_i RR_i (Qdot, :445-466)."""
+        n = out.shape[0]
+        _check_device("RR", RR, self.nS, RR.shape[1] if RR.dim() == 2 else -1, min_cols=n)
+        _check_device("Qdot", out.reshape(1, -1), 1, n, device=RR.device)
+        _capi.check(_capi.lib().gfc_qdot(self._h, n, RR.shape[1], _ptr(RR), _ptr(out),
+                                         self._stream(stream), 0))
 
     # ------------------------------------------------ reference interface
     def computeYNew(self, deltaT: float, deltaTChemMax: float, deltaTChem, Y):

@@ -72,6 +72,9 @@ typedef enum gfc_solver {
                                  device buffers owned by the evaluator */
 #define GFC_BATCHED 0x2       /* persistent work-stealing launch */
 #define GFC_GRAPH 0x4         /* replay the launch as a hipGraph (batched) */
+#define GFC_LANE_PER_CELL 0x8 /* without GFC_BATCHED: one lane per cell, the
+                                 reference's mapping (for_each_index.H:24-48),
+                                 in place of the wavefront kernels */
 
 /* odeCoeffs of chemistryProperties (readGpuOdeInputs.H:8-26). */
 typedef struct gfc_ode_inputs {
@@ -162,6 +165,15 @@ int gfc_mechanism_species_index(const gfc_mechanism* mech, const char* name);
  * (readGpuOdeInputs.H:12-22; gpuChemistryModel.C:69 Treact default 0). */
 int gfc_ode_inputs_from_foam(const char* chemistry_properties_text,
                              gfc_ode_inputs* out, double* Treact);
+/* The switches OpenFOAM's chemistry-model base class reads from the same
+ * dictionary, which gpuChemistryModel uses as chemistry_ and
+ * deltaTChemMax_ (gpuChemistryModel.C:147,338,345): `chemistry on|off`
+ * (required), `initialChemicalTimeStep` (required, the first deltaTChem of
+ * every cell) and `maxChemicalTimeStep` (default great = 1/DBL_EPSILON).
+ * Outputs may be NULL. */
+int gfc_chemistry_switches_from_foam(const char* chemistry_properties_text,
+                                     int* chemistry, double* initialChemicalTimeStep,
+                                     double* maxChemicalTimeStep);
 /* "Rosenbrock12|23|34" -> gfc_solver (makeGpuOdeSolver.H:13-31). */
 int gfc_solver_from_name(const char* name, int* solver);
 /* Defaults of readGpuOdeInputs.H:12-22 for the given solver. */
@@ -194,7 +206,14 @@ void gfc_evaluator_destroy(gfc_evaluator* ev);
  *                              accepted steps per cell
  * Pointers are device pointers unless GFC_HOST_POINTERS is set.  Without
  * GFC_HOST_POINTERS the call is asynchronous on `stream` and allocates
- * nothing. */
+ * nothing.
+ * Kernels: without GFC_BATCHED one launch covers the cells (the drop-in
+ * form): one wavefront per cell (nSpecie+1 <= 64; four cells per wavefront
+ * where the batched shape chooses that), one 512-thread workgroup per cell
+ * (64 < nSpecie+1 <= 208) on a resident grid dealing cells round-robin, or
+ * with GFC_LANE_PER_CELL one lane per cell.  GFC_BATCHED runs the same
+ * per-cell kernels over a persistent work-stealing grid; both forms give
+ * bitwise-identical results. */
 int gfc_solve(gfc_evaluator* ev, double deltaT, double deltaTChemMax,
               int nCells, int ldY, const double* rho, const double* Y0,
               double* Ynew, double* RR, double* deltaTChem, double* deltaTMin,
@@ -215,6 +234,42 @@ int gfc_evaluator_set_batched_shape(gfc_evaluator* ev, int cellsPerWave,
                                     int wavesPerBlock, int chunk);
 int gfc_evaluator_get_batched_shape(const gfc_evaluator* ev, int* cellsPerWave,
                                     int* wavesPerBlock, int* chunk);
+
+/* ---- chemistry queries at the current state ----------------------------
+ * The non-ODE queries of gpuFoam's chemistry model, which the reference
+ * evaluates on the CPU with OpenFOAM's Reaction objects
+ * (gpuChemistryModel.C:146-466).  Y is the SoA state of gfc_solve (T in
+ * row nSpecie, p in row nSpecie+1), rho the caller's density field, and
+ * c_i = rho Y_i / W_i (not clipped).  Rates are gpuReaction::omega's:
+ * constants at T clipped to the reaction limits, kr = kf/Kc.  Device
+ * pointers, asynchronous on `stream`, unless GFC_HOST_POINTERS is set.
+ * nCells <= the evaluator's maxCells; reaction indices are in reaction
+ * order of the dictionary.
+ *
+ * gfc_calculate: gpuChemistryModel::calculate (:146-183),
+ *   RR [nSpecie*ldY]: W_i sum_r nu_ir omega_r [kg/m^3/s].
+ * gfc_reaction_rr: reactionRR(reaction) (:186-240), rr [nCells]: omega_r
+ *   [kmol/m^3/s].
+ * gfc_specie_reaction_rr: specieReactionRR(reaction) (:242-317),
+ *   RR [nSpecie*ldY]: W_i nu_ir omega_r.
+ * gfc_tc: tc() (:366-443), tc [nCells]: sum(c) sum_r(wf_r + wr_r) /
+ *   sum_r(wf_r^2 + wr_r^2) with wf_r = omegaf_r sum_rhs nu and
+ *   wr_r = omegar_r sum_lhs nu, or DBL_MAX/10 when every rate is zero.
+ * gfc_qdot: Qdot() (:445-466), Qdot [nCells]: -sum_i hf_i RR_i for a
+ *   given RR [nSpecie*ldY] (e.g. the last gfc_solve's or gfc_calculate's),
+ *   hf_i the specie's enthalpy of formation per unit mass. */
+int gfc_calculate(gfc_evaluator* ev, int nCells, int ldY, const double* rho,
+                  const double* Y, double* RR, void* stream, unsigned flags);
+int gfc_reaction_rr(gfc_evaluator* ev, int reaction, int nCells, int ldY,
+                    const double* rho, const double* Y, double* rr, void* stream,
+                    unsigned flags);
+int gfc_specie_reaction_rr(gfc_evaluator* ev, int reaction, int nCells, int ldY,
+                           const double* rho, const double* Y, double* RR,
+                           void* stream, unsigned flags);
+int gfc_tc(gfc_evaluator* ev, int nCells, int ldY, const double* rho,
+           const double* Y, double* tc, void* stream, unsigned flags);
+int gfc_qdot(gfc_evaluator* ev, int nCells, int ldY, const double* RR,
+             double* Qdot, void* stream, unsigned flags);
 
 /* ---- rate parameters ----------------------------------------------------
  * No counterpart in the reference (its rates are fixed at construction,
@@ -237,7 +292,8 @@ int gfc_evaluator_set_rate_parameters(gfc_evaluator* ev, const double* params);
  * no deltaTMin, deltaTChem not clamped) that records the accepted steps.
  * For nSpecie+1 <= 16 it takes the batched form's choice of four cells per
  * wavefront (gfc_evaluator_set_batched_shape) and then returns the batched
- * solve's results; otherwise it returns the plain solve's.  The workspace
+ * solve's results; otherwise it returns the one-lane solve's
+ * (GFC_LANE_PER_CELL).  The workspace
  * (step sizes, checkpoints every K accepted steps, per-cell scratch) is
  * allocated by this call when it grows.
  *

@@ -910,3 +910,131 @@ int orc_solve_cells(const gfc_mech_view* m, const gfc_ode_inputs* ode,
     if (deltaTMin) *deltaTMin = minDt;
     return 0;
 }
+
+/* ---------------------------------------------------------- queries --- */
+
+/* OpenFOAM's Reaction::omega(p, T, c, li, omegaf, omegar), the function
+ * gpuChemistryModel's CPU queries call; its net value is gpuReaction::omega
+ * (gpuReaction.H:472-488), which the reference's tests hold equal to
+ * OpenFOAM's (Test-gpuChemistry.C:384-388). */
+void orc_reaction_omega_fr(const gfc_mech_view* m, int r, double p, double T,
+                           const double* c, double out[2]) {
+    (void)p;
+    const double Tc = fmin(fmax(T, m->Tlow), m->Thigh);
+    const double kf = rate_k(m, r, Tc, c);
+    const double kr = m->irreversible[r] ? 0.0 : kf / reaction_Kc_only(m, r, Tc);
+    double pw[WMAX_];
+    const double Cf = side_powers(m, r, 0, c, pw);
+    const double Cr = side_powers(m, r, 1, c, pw);
+    out[0] = kf * Cf;
+    out[1] = kr * Cr;
+}
+
+/* the concentrations the queries use: c_i = rho Y_i / W_i
+ * (gpuChemistryModel.C:168-171) */
+static void query_c(const gfc_mech_view* m, int cell, int ldY, const double* rho,
+                    const double* Y, double* c) {
+    for (int i = 0; i < m->n_species; ++i)
+        c[i] = rho[cell] * Y[(long)i * ldY + cell] / m->W[i];
+}
+
+/* gpuChemistryModel::calculate, gpuChemistryModel.C:146-183 */
+void orc_calculate(const gfc_mech_view* m, int nCells, int ldY,
+                   const double* rho, const double* Y, double* RR) {
+    const int nS = m->n_species;
+    double* c = (double*)malloc(sizeof(double) * 2 * (size_t)(nS + 2));
+    double* dN = c + nS + 2;
+    for (int cell = 0; cell < nCells; ++cell) {
+        const double T = Y[(long)nS * ldY + cell];
+        const double p = Y[(long)(nS + 1) * ldY + cell];
+        query_c(m, cell, ldY, rho, Y, c);
+        for (int i = 0; i < nS + 2; ++i) dN[i] = 0.0;
+        for (int r = 0; r < m->n_reactions; ++r) {
+            double out[3];
+            orc_reaction_rates(m, r, p, T, c, out);
+            add_dNdtByV(m, r, out[2], dN);
+        }
+        for (int i = 0; i < nS; ++i) RR[(long)i * ldY + cell] = dN[i] * m->W[i];
+    }
+    free(c);
+}
+
+/* gpuChemistryModel::reactionRR, gpuChemistryModel.C:186-240 */
+void orc_reaction_rr(const gfc_mech_view* m, int r, int nCells, int ldY,
+                     const double* rho, const double* Y, double* out) {
+    const int nS = m->n_species;
+    double* c = (double*)malloc(sizeof(double) * (size_t)(nS + 2));
+    for (int cell = 0; cell < nCells; ++cell) {
+        query_c(m, cell, ldY, rho, Y, c);
+        double o[3];
+        orc_reaction_rates(m, r, Y[(long)(nS + 1) * ldY + cell],
+                           Y[(long)nS * ldY + cell], c, o);
+        out[cell] = o[2];
+    }
+    free(c);
+}
+
+/* gpuChemistryModel::specieReactionRR, gpuChemistryModel.C:242-317 */
+void orc_specie_reaction_rr(const gfc_mech_view* m, int r, int nCells,
+                            int ldY, const double* rho, const double* Y,
+                            double* RR) {
+    const int nS = m->n_species;
+    double* c = (double*)malloc(sizeof(double) * 2 * (size_t)(nS + 2));
+    double* dN = c + nS + 2;
+    for (int cell = 0; cell < nCells; ++cell) {
+        query_c(m, cell, ldY, rho, Y, c);
+        for (int i = 0; i < nS + 2; ++i) dN[i] = 0.0;
+        double o[3];
+        orc_reaction_rates(m, r, Y[(long)(nS + 1) * ldY + cell],
+                           Y[(long)nS * ldY + cell], c, o);
+        add_dNdtByV(m, r, o[2], dN);
+        for (int i = 0; i < nS; ++i) RR[(long)i * ldY + cell] = dN[i] * m->W[i];
+    }
+    free(c);
+}
+
+/* gpuChemistryModel::tc, gpuChemistryModel.C:366-443: the reciprocal of the
+ * reactions' molar production rates per total mole, weighted by those
+ * rates, forward and reverse parts taken separately */
+void orc_tc(const gfc_mech_view* m, int nCells, int ldY, const double* rho,
+            const double* Y, double* tc) {
+    const int nS = m->n_species;
+    double* c = (double*)malloc(sizeof(double) * (size_t)(nS + 2));
+    for (int cell = 0; cell < nCells; ++cell) {
+        const double T = Y[(long)nS * ldY + cell];
+        const double p = Y[(long)(nS + 1) * ldY + cell];
+        query_c(m, cell, ldY, rho, Y, c);
+        double sumW = 0, sumWRateByCTot = 0;
+        for (int r = 0; r < m->n_reactions; ++r) {
+            double o[2];
+            orc_reaction_omega_fr(m, r, p, T, c, o);
+            double wf = 0;
+            for (int s = 0; s < m->n_rhs[r]; ++s)
+                wf += m->rhs_stoich[(long)r * WMAX_ + s] * o[0];
+            sumW += wf;
+            sumWRateByCTot += wf * wf;
+            double wr = 0;
+            for (int s = 0; s < m->n_lhs[r]; ++s)
+                wr += m->lhs_stoich[(long)r * WMAX_ + s] * o[1];
+            sumW += wr;
+            sumWRateByCTot += wr * wr;
+        }
+        double ctot = 0;
+        for (int i = 0; i < nS; ++i) ctot += c[i];
+        tc[cell] = sumWRateByCTot == 0 ? VGREAT_ : sumW / sumWRateByCTot * ctot;
+    }
+    free(c);
+}
+
+/* gpuChemistryModel::Qdot, gpuChemistryModel.C:445-466, with
+ * hf = gpuThermo::Hf (the reference's tests hold it equal to OpenFOAM's
+ * hf(), Test-gpuChemistry.C:173-180) */
+void orc_qdot(const gfc_mech_view* m, int nCells, int ldY, const double* RR,
+              double* Qdot) {
+    for (int cell = 0; cell < nCells; ++cell) Qdot[cell] = 0.0;
+    for (int i = 0; i < m->n_species; ++i) {
+        const double hi = poly_Hf(m->low + 7 * i);
+        for (int cell = 0; cell < nCells; ++cell)
+            Qdot[cell] -= hi * RR[(long)i * ldY + cell];
+    }
+}

@@ -77,6 +77,30 @@ int orc_solve_cells(const gfc_mech_view* m, const gfc_ode_inputs* ode,
                     double* Ynew, double* RR, double* deltaTChem,
                     double* deltaTMin, int* cellStats, int nThreads);
 
+/* --- chemistry queries at the current state (gpuChemistryModel.C:146-466)
+ * Y: SoA [(nS+2)*ldY], T at row nS, p at row nS+1; rho [nCells], the
+ * thermo's density; c_i = rho Y_i / W_i, not clipped. */
+/* OpenFOAM's Reaction::omega(p, T, c, li, omegaf, omegar) split:
+ * out[0] = kf Cf, out[1] = kr Cr (rates at the clipped T) */
+void orc_reaction_omega_fr(const gfc_mech_view* m, int r, double p, double T,
+                           const double* c, double out[2]);
+/* calculate(), :146-183: RR[i*ldY+cell] = W_i dNdtByV_i */
+void orc_calculate(const gfc_mech_view* m, int nCells, int ldY,
+                   const double* rho, const double* Y, double* RR);
+/* reactionRR(r), :186-240: out[cell] = omega_r */
+void orc_reaction_rr(const gfc_mech_view* m, int r, int nCells, int ldY,
+                     const double* rho, const double* Y, double* out);
+/* specieReactionRR(r), :242-317: RR[i*ldY+cell] = W_i nu_ir omega_r */
+void orc_specie_reaction_rr(const gfc_mech_view* m, int r, int nCells,
+                            int ldY, const double* rho, const double* Y,
+                            double* RR);
+/* tc(), :366-443 */
+void orc_tc(const gfc_mech_view* m, int nCells, int ldY, const double* rho,
+            const double* Y, double* tc);
+/* Qdot(), :445-466: Qdot[cell] = -sum_i hf_i RR_i */
+void orc_qdot(const gfc_mech_view* m, int nCells, int ldY, const double* RR,
+              double* Qdot);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,12 @@ def oracle() -> C.CDLL:
         L.orc_solve_cells.argtypes = [_MV, _OI, C.c_double, C.c_double, C.c_double,
                                       C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
                                       _ip, C.c_int]
+        L.orc_reaction_omega_fr.argtypes = [_MV, C.c_int, C.c_double, C.c_double, _dp, _dp]
+        for name in ("orc_calculate", "orc_tc"):
+            getattr(L, name).argtypes = [_MV, C.c_int, C.c_int, _dp, _dp, _dp]
+        for name in ("orc_reaction_rr", "orc_specie_reaction_rr"):
+            getattr(L, name).argtypes = [_MV, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.orc_qdot.argtypes = [_MV, C.c_int, C.c_int, _dp, _dp]
         _oracle = L
     return _oracle
 
@@ -158,6 +164,52 @@ class Oracle:
         st = self.L.orc_ode_solve(self.v, C.byref(ode), x0, x1, dptr(y), dptr(dx),
                                   dptr(self.work), iptr(stats))
         return y, dx[0], st, stats
+
+    # chemistry queries (gpuChemistryModel.C:146-466); Y [nS+2, n] SoA
+    def omega_fr(self, r, p, T, c):
+        out = np.zeros(2)
+        self.L.orc_reaction_omega_fr(self.v, r, p, T, dptr(np.ascontiguousarray(c, float)),
+                                     dptr(out))
+        return out
+
+    def calculate(self, rho, Y):
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        n = Y.shape[1]
+        RR = np.zeros((self.nS, n))
+        self.L.orc_calculate(self.v, n, n, dptr(np.ascontiguousarray(rho, float)), dptr(Y),
+                             dptr(RR))
+        return RR
+
+    def tc(self, rho, Y):
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        n = Y.shape[1]
+        out = np.zeros(n)
+        self.L.orc_tc(self.v, n, n, dptr(np.ascontiguousarray(rho, float)), dptr(Y),
+                      dptr(out))
+        return out
+
+    def reaction_rr(self, r, rho, Y):
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        n = Y.shape[1]
+        out = np.zeros(n)
+        self.L.orc_reaction_rr(self.v, r, n, n, dptr(np.ascontiguousarray(rho, float)),
+                               dptr(Y), dptr(out))
+        return out
+
+    def specie_reaction_rr(self, r, rho, Y):
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        n = Y.shape[1]
+        RR = np.zeros((self.nS, n))
+        self.L.orc_specie_reaction_rr(self.v, r, n, n, dptr(np.ascontiguousarray(rho, float)),
+                                      dptr(Y), dptr(RR))
+        return RR
+
+    def qdot(self, RR):
+        RR = np.ascontiguousarray(RR, dtype=np.float64)
+        n = RR.shape[1]
+        out = np.zeros(n)
+        self.L.orc_qdot(self.v, n, n, dptr(RR), dptr(out))
+        return out
 
     def solve_cells(self, ode, Y0, deltaT, dtChem, rho=None, Treact=0.0,
                     deltaTChemMax=1e300, threads=0, want_rr=False):

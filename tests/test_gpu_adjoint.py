@@ -1,5 +1,5 @@
 """GPU tests of the discrete adjoint through the C ABI: the recording
-forward is the plain solve, the device gradients equal the host build of
+forward is the one-lane solve, the device gradients equal the host build of
 the same adjoint code (itself checked against the oracle on the CPU), and
 they match central finite differences of the replayed accepted steps
 (relative step 1e-6), also through the test-only torch.autograd.Function."""
@@ -48,7 +48,7 @@ def test_recording_forward_is_the_plain_solve(config):
     m, Y0, dt0, c, ode = _case(config, 96)
     ev = Evaluator(m, ode, 0.0, max_cells=96)
     _, dYn, ddt, st = _forward(ev, c, Y0, dt0)
-    Yn, dt, _, stp, _ = ev.solve_host(c["deltaT"], Y0, dt0)
+    Yn, dt, _, stp, _ = ev.solve_host(c["deltaT"], Y0, dt0, lane=True)
     torch.cuda.synchronize()
     assert np.array_equal(dYn.cpu().numpy(), Yn)
     assert np.array_equal(ddt.cpu().numpy(), dt)

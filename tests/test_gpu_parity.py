@@ -1,9 +1,13 @@
 """Parity of the HIP kernels with the CPU oracle (MI355X).
 
-Both device forms -- the plain per-cell kernel (one lane per cell, the
-reference's mapping) and the batched persistent wavefront-per-cell kernel --
-are compared with the oracle on the same seeded states:
-
+Three device forms are compared with the oracle on the same seeded states:
+  * lane     one lane per cell (GFC_LANE_PER_CELL), the reference's mapping
+             and operation order: identical step counts to the oracle;
+  * plain    the drop-in launch (no GFC_BATCHED): the wavefront, 16-lane-team
+             or workgroup kernel over one grid of cells;
+  * batched  the same kernels over a persistent work-stealing grid; its
+             results equal the plain form's bit for bit.
+Bounds:
   species  |Y_gpu - Y_cpu| <= 1e-6 |Y_cpu| + 1e-10      (rtol 1e-6, absTol)
   T        |T_gpu - T_cpu| <= 1e-8 T_cpu
   RR       from the same Ynew, with rho
@@ -44,7 +48,7 @@ def _oracle_run(cfg, n, **kw):
 
 
 def _device_run(m, Y0, dt0, c, ode, rho, batched, graph=False, deltaTChemMax=1e300,
-                Treact=0.0, cells_per_wave=None):
+                Treact=0.0, cells_per_wave=None, lane=False):
     from gpufoam_amd.evaluator import Evaluator
     n = Y0.shape[1]
     ev = Evaluator(m, ode, Treact, max_cells=n)
@@ -58,7 +62,7 @@ def _device_run(m, Y0, dt0, c, ode, rho, batched, graph=False, deltaTChemMax=1e3
     st = torch.zeros((3, n), dtype=torch.int32, device="cuda")
     drho = torch.tensor(rho, device="cuda")
     ev.solve(c["deltaT"], dY0, ddt, Ynew=dYn, RR=dRR, rho=drho, deltaTMin=dmin, stats=st,
-             batched=batched, graph=graph, deltaTChemMax=deltaTChemMax)
+             batched=batched, graph=graph, deltaTChemMax=deltaTChemMax, lane=lane)
     torch.cuda.synchronize()
     return (dYn.cpu().numpy(), ddt.cpu().numpy(), dmin.item(), st.cpu().numpy(),
             dRR.cpu().numpy())
@@ -93,28 +97,37 @@ def _check(m, got, ref, tight_dt, Y0=None, rho=None, deltaT=None, dt_rtol=1e-6,
         assert np.all(np.abs(rrg - rro) <= 1.01 * bound)
 
 
-CASES = [("h2_small", 384), ("gri_production", 192), ("gri_batching", 32), ("h2_adjoint", 48)]
+# lane kernel: one wavefront holds 64 cells, so its samples stay small
+LANE_CASES = [("h2_small", 384), ("gri_production", 192), ("gri_batching", 32),
+              ("h2_adjoint", 48)]
+CASES = [("h2_small", 384), ("gri_production", 2048), ("gri_batching", 512),
+         ("h2_adjoint", 48)]
 
 
-@pytest.mark.parametrize("cfg,n", CASES)
-def test_plain_kernel_matches_oracle(cfg, n):
-    print(f"case {cfg} n={n} plain", flush=True)
+@pytest.mark.parametrize("cfg,n", LANE_CASES)
+def test_lane_kernel_matches_oracle(cfg, n):
+    print(f"case {cfg} n={n} lane", flush=True)
     m, Y0, dt0, c, ode, rho, ref = _oracle_run(cfg, n)
-    got = _device_run(m, Y0, dt0, c, ode, rho, batched=False)
+    got = _device_run(m, Y0, dt0, c, ode, rho, batched=False, lane=True)
     _check(m, got, ref, tight_dt=True, Y0=Y0, rho=rho, deltaT=c["deltaT"])
 
 
 @pytest.mark.parametrize("cfg,n", CASES)
-def test_batched_kernel_matches_oracle(cfg, n):
-    print(f"case {cfg} n={n} batched", flush=True)
+def test_plain_and_batched_match_oracle(cfg, n):
+    """The drop-in launch against the oracle, and the persistent launch of
+    the same kernel equal to it bit for bit."""
+    print(f"case {cfg} n={n} plain + batched", flush=True)
     m, Y0, dt0, c, ode, rho, ref = _oracle_run(cfg, n)
-    got = _device_run(m, Y0, dt0, c, ode, rho, batched=True)
+    got = _device_run(m, Y0, dt0, c, ode, rho, batched=False)
     _check(m, got, ref, tight_dt=False, Y0=Y0, rho=rho, deltaT=c["deltaT"])
+    bat = _device_run(m, Y0, dt0, c, ode, rho, batched=True)
+    for a, b in zip(got, bat):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
 
 
 @pytest.mark.parametrize("solver", ["Rosenbrock12", "Rosenbrock23"])
-@pytest.mark.parametrize("batched", [False, True])
-def test_other_solvers(solver, batched):
+@pytest.mark.parametrize("form", ["lane", "plain", "batched"])
+def test_other_solvers(solver, form):
     """Ros12 and Ros23 on H2 states with the reference tutorial tolerances
     (tutorials/h2: absTol 1e-12, relTol 0.1 -- here relTol 1e-5)."""
     from oracle_lib import Oracle
@@ -122,8 +135,9 @@ def test_other_solvers(solver, batched):
     ode = ode_inputs(solver, relTol=1e-5, absTol=1e-10)
     rho = np.ones(128)
     ref = Oracle(m).solve_cells(ode, Y0, 1e-5, dt0, rho=rho, want_rr=True)
-    got = _device_run(m, Y0, dt0, c | {"deltaT": 1e-5}, ode, rho, batched=batched)
-    _check(m, got, ref, tight_dt=not batched)
+    got = _device_run(m, Y0, dt0, c | {"deltaT": 1e-5}, ode, rho,
+                      batched=form == "batched", lane=form == "lane")
+    _check(m, got, ref, tight_dt=form == "lane")
 
 
 @pytest.mark.parametrize("solver", ["Rosenbrock12", "Rosenbrock23"])
@@ -198,11 +212,11 @@ def test_batched_is_deterministic_and_graph_replays():
 
 
 def test_plain_lane_shape_equals_plain():
-    """The batched launcher with one lane per cell runs the plain kernel."""
+    """The batched launcher with one lane per cell runs the lane kernel."""
     from gpufoam_amd.evaluator import Evaluator
     m, Y0, dt0, c = make_states("h2_small", 100)
     ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
-    a = _device_run(m, Y0, dt0, c, ode, np.ones(100), batched=False)
+    a = _device_run(m, Y0, dt0, c, ode, np.ones(100), batched=False, lane=True)
     ev = Evaluator(m, ode, 0.0, max_cells=100)
     ev.set_batched_shape(64)
     Yn, dt, dmin, st, _ = ev.solve_host(c["deltaT"], Y0, dt0, batched=True)
@@ -218,10 +232,10 @@ def test_treact_gate_and_clamp():
     nS = m.n_species
     Treact = 1200.0
     ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
-    for batched in (False, True):
+    for batched, lane in ((False, True), (False, False), (True, False)):
         ev = Evaluator(m, ode, Treact, max_cells=64)
         Yn, dt, dmin, st, rr = ev.solve_host(c["deltaT"], Y0, dt0, deltaTChemMax=2e-6,
-                                             want_rr=True, batched=batched)
+                                             want_rr=True, batched=batched, lane=lane)
         cold = Y0[nS] <= Treact
         assert cold.any() and (~cold).any()
         assert np.array_equal(Yn[:, cold], Y0[:, cold])
@@ -283,11 +297,30 @@ def test_max_steps_is_reported():
     m, Y0, dt0, c = make_states("h2_small", 32)
     ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"], maxSteps=3)
     ref = Oracle(m).solve_cells(ode, Y0, c["deltaT"], dt0)
-    for batched in (False, True):
+    for batched, lane in ((False, True), (False, False), (True, False)):
         ev = Evaluator(m, ode, 0.0, max_cells=32)
-        Yn, dt, _, st, _ = ev.solve_host(c["deltaT"], Y0, dt0, batched=batched)
+        Yn, dt, _, st, _ = ev.solve_host(c["deltaT"], Y0, dt0, batched=batched, lane=lane)
         assert np.array_equal(st[0], ref[3][0])
         assert (st[0] & 1).any()
+
+
+def test_step_underflow_is_reported():
+    """With absTol = relTol = 0 every error estimate is infinite, so
+    adaptiveSolve shrinks the step by minScale until dx < vSmall
+    (gpuAdaptiveSolver.H:111-114, where the reference prints and asserts);
+    here the cell stops with GFC_CELL_STEP_UNDERFLOW, as in the oracle."""
+    from gpufoam_amd.evaluator import Evaluator
+    from oracle_lib import Oracle
+    m, Y0, dt0, c = make_states("h2_small", 64)
+    ode = ode_inputs(c["solver"], relTol=0.0, absTol=0.0)
+    ref = Oracle(m).solve_cells(ode, Y0, c["deltaT"], dt0)
+    assert (ref[3][0] & 2).all()
+    for batched, lane in ((False, True), (False, False), (True, False)):
+        print(f"case h2_small n=64 underflow batched={batched} lane={lane}", flush=True)
+        ev = Evaluator(m, ode, 0.0, max_cells=64)
+        Yn, dt, _, st, _ = ev.solve_host(c["deltaT"], Y0, dt0, batched=batched, lane=lane)
+        assert np.array_equal(st[0], ref[3][0])
+        assert np.array_equal(st[1], ref[3][1]) and np.array_equal(st[2], ref[3][2])
 
 
 def test_synthetic_200_species_matches_oracle():

@@ -379,3 +379,28 @@ def test_counted_work_constants_are_reproducible():
         with open(os.path.join(root, "gpufoam_amd", "flops", f"{mech}.json")) as f:
             assert json.load(f) == d, mech
     assert "syn200" in out
+
+
+def test_step_underflow_stops_the_cell():
+    """absTol = relTol = 0 makes every normalised error infinite, so
+    adaptiveSolve keeps shrinking the step by minScale
+    (gpuAdaptiveSolver.H:99-114) until dx < vSmall, where the reference
+    prints and asserts.  The oracle reports GFC_CELL_STEP_UNDERFLOW instead,
+    after exactly the attempts that take the first step below DBL_MIN, with
+    no step accepted and deltaTChem left at its initial value."""
+    m, Y0, dt0, c = make_states("h2_small", 16)
+    ode = ode_inputs(c["solver"], relTol=0.0, absTol=0.0)
+    Yn, dt, _, st, _ = Oracle(m).solve_cells(ode, Y0, c["deltaT"], dt0)
+    # attempts: dx0 * minScale^k first below DBL_MIN (binary64 products)
+    dx, k = float(dt0[0]), 0
+    while True:
+        k += 1
+        dx *= ode.minScale
+        if dx < np.finfo(np.float64).tiny:
+            break
+    assert (st[0] == 2).all()
+    assert (st[1] == k).all() and (st[2] == 0).all()
+    nS = m.n_species
+    assert np.array_equal(Yn[:nS], np.maximum(Y0[:nS], 0.0))
+    assert np.array_equal(Yn[nS:], Y0[nS:])
+    assert np.array_equal(dt, dt0)

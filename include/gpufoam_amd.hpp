@@ -163,6 +163,138 @@ private:
     int cap_ = 0;
 };
 
+// Thrown where the reference throws std::logic_error: the LTS solve
+// (gpuChemistryModel.C:328-333).  Carries the library's status code.
+struct LogicError : std::logic_error {
+    int status;
+    LogicError(int s, const std::string& what) : std::logic_error(what), status(s) {}
+};
+
+// The surface of gpuFoam's chemistry-model plugin
+// (gpuChemistryModel.H:26-185, gpuChemistryModel.C) without OpenFOAM: the
+// model owns RR (per specie, [nSpecie][nCells]) and deltaTChem
+// ([nCells]), reads its switches from chemistryProperties, and takes the
+// cell fields as structure-of-arrays host arrays, one row per field
+// (Y_0..Y_nS-1, T, p), which is how an OpenFOAM adapter holds them (one
+// volScalarField per specie).  An adapter forwards its virtual overrides
+// here: solve(deltaT) with the old-time fields, calculate/tc/reactionRR/
+// specieReactionRR with the current ones.
+class ChemistryModel {
+public:
+    ChemistryModel(const Mechanism& mech, const std::string& chemistryProperties, int nCells)
+        : mech_(&mech), nCells_(nCells) {
+        if (nCells < 0)
+            throw Error(GFC_ERR_INVALID_ARGUMENT, "ChemistryModel: negative cell count");
+        const gfc_mech_view v = mech.view();
+        nSpecie_ = v.n_species;
+        nReaction_ = v.n_reactions;
+        int on = 0;
+        double dt0 = 0.0;
+        check(gfc_chemistry_switches_from_foam(chemistryProperties.c_str(), &on, &dt0,
+                                               &deltaTChemMax_));
+        chemistry_ = on != 0;
+        check(gfc_ode_inputs_from_foam(chemistryProperties.c_str(), &inputs_, &Treact_));
+        check(gfc_evaluator_create(mech.get(), &inputs_, Treact_, std::max(nCells, 1), &ev_));
+        RR_.assign((size_t)nSpecie_ * nCells, 0.0);
+        deltaTChem_.assign(nCells, dt0);
+    }
+    ChemistryModel(const ChemistryModel&) = delete;
+    ChemistryModel& operator=(const ChemistryModel&) = delete;
+    ~ChemistryModel() { gfc_evaluator_destroy(ev_); }
+
+    int nSpecie() const { return nSpecie_; }
+    int nReaction() const { return nReaction_; }
+    int nCells() const { return nCells_; }
+    int nEqns() const { return nSpecie_ + 2; }
+    bool chemistry() const { return chemistry_; }
+    double Treact() const { return Treact_; }
+    double deltaTChemMax() const { return deltaTChemMax_; }
+    // the reaction string as written in the dictionary
+    std::string reactionName(int r) const {
+        const char* s = gfc_mechanism_reaction_equation(mech_->get(), r);
+        if (!s) throw Error(GFC_ERR_INVALID_ARGUMENT, "reactionName: index out of range");
+        return s;
+    }
+    // RR[i*nCells + cell] [kg/m^3/s] of the last solve or calculate
+    const std::vector<double>& RR() const { return RR_; }
+    const std::vector<double>& deltaTChem() const { return deltaTChem_; }
+
+    // solve(deltaT) (gpuChemistryModel.C:320-326): one chemistry step of
+    // every cell from the old-time fields (rho0 [nCells], Y0 SoA
+    // [(nSpecie+2)*nCells]); stores RR and the clamped deltaTChem and
+    // returns min(deltaTMin, 2 deltaT), deltaTMin taken before clamping
+    // (computeRRAndChemDeltaT, :336-364).  With chemistry off nothing is
+    // computed and the step limit is min(great, 2 deltaT).
+    double solve(double deltaT, const double* rho0, const double* Y0) {
+        double dtMin = kGreat;
+        if (chemistry_ && nCells_ > 0)
+            check(gfc_solve(ev_, deltaT, deltaTChemMax_, nCells_, nCells_, rho0, Y0, nullptr,
+                            RR_.data(), deltaTChem_.data(), &dtMin, nullptr, nullptr,
+                            GFC_HOST_POINTERS | GFC_BATCHED));
+        return std::min(dtMin, 2 * deltaT);
+    }
+    // solve(scalarField) (gpuChemistryModel.C:328-333): local time stepping
+    // is not supported, as in the reference.
+    double solve(const std::vector<double>& deltaT, const double* rho0, const double* Y0) {
+        (void)deltaT;
+        (void)rho0;
+        (void)Y0;
+        throw LogicError(GFC_ERR_LTS, "LTS time step not yet supported");
+    }
+
+    // calculate() (:146-183): RR from the current state.
+    void calculate(const double* rho, const double* Y) {
+        if (!chemistry_ || nCells_ == 0) return;
+        check(gfc_calculate(ev_, nCells_, nCells_, rho, Y, RR_.data(), nullptr,
+                            GFC_HOST_POINTERS));
+    }
+    // tc() (:366-443) [s]; `small` in every cell with chemistry off.
+    std::vector<double> tc(const double* rho, const double* Y) const {
+        std::vector<double> t(nCells_, kSmall);
+        if (chemistry_ && nCells_ > 0)
+            check(gfc_tc(ev_, nCells_, nCells_, rho, Y, t.data(), nullptr, GFC_HOST_POINTERS));
+        return t;
+    }
+    // Qdot() (:445-466) [W/m^3] from the stored RR.
+    std::vector<double> Qdot() const {
+        std::vector<double> q(nCells_, 0.0);
+        if (chemistry_ && nCells_ > 0)
+            check(gfc_qdot(ev_, nCells_, nCells_, RR_.data(), q.data(), nullptr,
+                           GFC_HOST_POINTERS));
+        return q;
+    }
+    // reactionRR(r) (:186-240) [kmol/m^3/s].
+    std::vector<double> reactionRR(int r, const double* rho, const double* Y) const {
+        std::vector<double> out(nCells_, 0.0);
+        if (chemistry_ && nCells_ > 0)
+            check(gfc_reaction_rr(ev_, r, nCells_, nCells_, rho, Y, out.data(), nullptr,
+                                  GFC_HOST_POINTERS));
+        return out;
+    }
+    // specieReactionRR(r) (:242-317), [nSpecie][nCells] [kg/m^3/s].
+    std::vector<double> specieReactionRR(int r, const double* rho, const double* Y) const {
+        std::vector<double> out((size_t)nSpecie_ * nCells_, 0.0);
+        if (chemistry_ && nCells_ > 0)
+            check(gfc_specie_reaction_rr(ev_, r, nCells_, nCells_, rho, Y, out.data(), nullptr,
+                                         GFC_HOST_POINTERS));
+        return out;
+    }
+
+    gfc_evaluator* handle() { return ev_; }
+
+    static constexpr double kGreat = 1.0 / 2.220446049250313e-16; // 1/DBL_EPSILON
+    static constexpr double kSmall = 2.220446049250313e-16;       // DBL_EPSILON
+
+private:
+    const Mechanism* mech_;
+    int nCells_ = 0, nSpecie_ = 0, nReaction_ = 0;
+    bool chemistry_ = true;
+    double Treact_ = 0.0, deltaTChemMax_ = 0.0;
+    gfc_ode_inputs inputs_{};
+    gfc_evaluator* ev_ = nullptr;
+    std::vector<double> RR_, deltaTChem_;
+};
+
 } // namespace gpufoam_amd
 
 #endif /* GPUFOAM_AMD_HPP */

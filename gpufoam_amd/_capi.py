@@ -94,6 +94,7 @@ def lib() -> C.CDLL:
             "gfc_mechanism_species_index": (C.c_int, [vp, C.c_char_p]),
             "gfc_ode_inputs_from_foam": (C.c_int, [C.c_char_p, C.POINTER(OdeInputs), _dp]),
             "gfc_solver_from_name": (C.c_int, [C.c_char_p, _ip]),
+            "gfc_chemistry_switches_from_foam": (C.c_int, [C.c_char_p, _ip, _dp, _dp]),
             "gfc_ode_inputs_default": (None, [C.c_int, C.POINTER(OdeInputs)]),
             "gfc_evaluator_create": (C.c_int, [vp, C.POINTER(OdeInputs), C.c_double,
                                               C.c_int, C.POINTER(vp)]),
@@ -118,6 +119,8 @@ def lib() -> C.CDLL:
             "gfc_specie_reaction_rr": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp,
                                                  vp, C.c_uint]),
             "gfc_qdot": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, C.c_uint]),
+            "gfc_solve_noalloc": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                            vp, vp, vp, vp, vp, vp, vp, vp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name, None)
@@ -143,6 +146,7 @@ def exported_symbols() -> list[str]:
         "gfc_adjoint_set_checkpoint_interval", "gfc_adjoint_checkpoint_interval",
         "gfc_adjoint_forward", "gfc_adjoint_backward", "gfc_adjoint_replay",
         "gfc_calculate", "gfc_tc", "gfc_reaction_rr", "gfc_specie_reaction_rr", "gfc_qdot",
+        "gfc_chemistry_switches_from_foam", "gfc_solve_noalloc",
     ]
 
 

@@ -247,6 +247,26 @@ int plain_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s, bool lane
     return hipFail(hipGetLastError(), "drop-in launch") ? GFC_ERR_HIP : GFC_OK;
 }
 
+int batched_launch_queue(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s, int* queue) {
+    Plan pl;
+    int perBlock = 1;
+    choose_plan(ev, A.nCells, pl, perBlock);
+    if (!pl.team) {
+        setLastError("the persistent form of this mechanism needs the workgroup-per-cell or "
+                     "one-lane-per-cell kernel, whose per-cell scratch the caller-free "
+                     "entry point does not hold (wavefront kernels only: nSpecie+1 <= 64)");
+        return GFC_ERR_CAPACITY;
+    }
+    SolveArgs At = A;
+    At.m = ev->dmTeam;
+    const int blocksNeeded = (A.nCells + perBlock - 1) / perBlock;
+    int grid = pl.grid < blocksNeeded ? pl.grid : blocksNeeded;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, s, queue);
+    hipLaunchKernelGGL(pl.team, dim3(grid), dim3(64), 0, s, At, queue);
+    return hipFail(hipGetLastError(), "persistent launch") ? GFC_ERR_HIP : GFC_OK;
+}
+
 int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
                    bool graph) {
     auto* b = static_cast<BatchedState*>(ev->batched);

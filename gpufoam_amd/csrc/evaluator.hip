@@ -398,6 +398,12 @@ int gfc_evaluator_create(const gfc_mechanism* mech, const gfc_ode_inputs* ode,
     ev->ode = *ode;
     ev->Treact = Treact;
     hipGetDevice(&ev->device);
+    // the caller-free entry point's device array, resolved on this device
+    int sst = noalloc_init_device(ev->device);
+    if (sst != GFC_OK) {
+        delete ev;
+        return sst;
+    }
     ev->mech = *mech;
     int st = evaluator_upload(ev);
     if (st != GFC_OK) {

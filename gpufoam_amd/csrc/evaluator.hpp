@@ -55,6 +55,13 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
 // cell over a grid of nCells, four cells per wavefront, or the workgroup
 // kernel dealing cells round-robin; lane = true: one lane per cell.
 int plain_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s, bool lane);
+// The persistent wavefront kernels with a caller-given work-queue counter
+// (no graph); GFC_ERR_CAPACITY for mechanisms that need the workgroup or
+// one-lane kernels.
+int batched_launch_queue(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s, int* queue);
+// Resolves the caller-free entry point's device array on the current
+// device (once per device).
+int noalloc_init_device(int device);
 // Whether a launch of nCells cells runs four cells per wavefront (the
 // evaluator's shape, or the automatic choice); N: the kernels' size.
 bool team16_selected(const gfc_evaluator* ev, int nCells, int& N);

@@ -219,6 +219,27 @@ int gfc_solve(gfc_evaluator* ev, double deltaT, double deltaTChemMax,
               double* Ynew, double* RR, double* deltaTChem, double* deltaTMin,
               int* cellStats, void* stream, unsigned flags);
 
+/* Allocation-free variant of gfc_solve's persistent batched form.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory at the call (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Device pointers only, the same
+ * arguments and results as gfc_solve(..., GFC_BATCHED) without a graph.
+ * The persistent kernel's work-queue counter and the deltaTMin reduction
+ * word live in a library-static device array of 256 bytes per GPU
+ * (resolved when an evaluator is created on that GPU, held until the
+ * process ends) instead of the evaluator's buffers, and the call allocates
+ * nothing.  Limits: the wavefront kernels only (nSpecie+1 <= 64, or four
+ * cells per wavefront where the batched shape selects it); mechanisms that
+ * need the workgroup-per-cell or one-lane kernels (per-cell scratch beyond
+ * the array), or a batched shape of one lane per cell, return
+ * GFC_ERR_CAPACITY, as does nCells > maxCells.  Calls on one GPU share its
+ * array: calls that overlap in time on the same GPU race; serialise them
+ * (one stream, or events). */
+int gfc_solve_noalloc(gfc_evaluator* ev, double deltaT, double deltaTChemMax,
+                      int nCells, int ldY, const double* rho, const double* Y0,
+                      double* Ynew, double* RR, double* deltaTChem,
+                      double* deltaTMin, int* cellStats, void* stream);
+
 /* Launch-shape control for the persistent batched form (the autotuner's
  * search space, gpufoam_amd/autotune.py).  cellsPerWave: 0 = automatic
  * (by mechanism size: four cells per wavefront for nSpecie+1 <= 16 from

@@ -7,12 +7,14 @@ parsed by this package's mechanism compiler and written back in canonical
 OpenFOAM form (round-trip exact numbers); the script checks that the
 re-emitted text compiles to bit-identical tables.
 
-    python -m gpufoam_amd.mechanisms.import_reference [/path/to/gpuFoam]
+    python tests/golden/import_reference.py [/path/to/gpuFoam]
 """
 import os
 import sys
 
 import numpy as np
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", ".."))
 
 from gpufoam_amd.mechanism import MECH_DIR, Mechanism
 

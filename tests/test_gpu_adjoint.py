@@ -122,6 +122,48 @@ def test_gradients_match_replayed_central_differences(config, ncells, maxSteps):
     ev.set_rate_parameters(theta0)
 
 
+def test_syn200_gradients_match_replayed_central_differences():
+    """The 200-species synthetic mechanism (n = 201, one lane per cell in
+    the adjoint), four cells over a short CFD step (2e-9 s), gradients with
+    respect to the initial state and every rate parameter against central
+    differences of the replayed accepted steps (relative step 1e-6)."""
+    m, Y0, dt0, c, ode = _case("syn200", 4)
+    c = dict(c, deltaT=2e-9)
+    nS, n, nR = m.n_species, m.n_species + 1, m.n_reactions
+    print("case syn200 n=4 deltaT=2e-9 adjoint", flush=True)
+    ev = Evaluator(m, ode, 0.0, max_cells=4)
+    theta0 = ev.get_rate_parameters()
+    _, _, _, st = _forward(ev, c, Y0, dt0)
+    print(f"accepted steps {st.cpu().numpy()[2].tolist()}", flush=True)
+    rng = np.random.default_rng(5)
+    W = rng.standard_normal((n, 4))
+    Y0bar = torch.empty((n, 4), dtype=torch.float64, device=DEV)
+    pbar = torch.empty((6 * nR, 4), dtype=torch.float64, device=DEV)
+    ev.adjoint_backward(torch.tensor(W, device=DEV), Y0bar, pbar)
+    gY, gP = Y0bar.cpu().numpy(), pbar.cpu().numpy()
+    tscale = AH.theta_scale(m).reshape(nR, 6)
+    eps = 1e-6
+    for trial in range(2):
+        D = rng.standard_normal((n, 4)) * np.abs(Y0[:n])
+        Dth = rng.standard_normal((nR, 6)) * tscale * (trial == 1)
+
+        def L(sign):
+            Yp = Y0.copy()
+            Yp[:n] += sign * eps * D
+            ev.set_rate_parameters(theta0 + sign * eps * Dth)
+            dYp = torch.tensor(Yp, device=DEV)
+            out = torch.empty_like(dYp)
+            ev.adjoint_replay(dYp, out)
+            torch.cuda.synchronize()
+            return np.sum(W * out.cpu().numpy()[:n], axis=0)
+        fd = (L(1) - L(-1)) / (2 * eps)
+        ad = np.sum(gY * D, axis=0) + gP.T @ Dth.reshape(-1)
+        rel = np.abs(ad - fd) / np.abs(fd)
+        print(f"trial {trial}: max relative gradient error {rel.max():.3g}", flush=True)
+        assert rel.max() <= 1e-5, (trial, rel.max(), int(rel.argmax()))
+    ev.set_rate_parameters(theta0)
+
+
 def test_checkpoint_interval_is_invisible_on_device():
     m, Y0, dt0, c, ode = _case("h2_small", 64)
     n = m.n_species + 1

@@ -1,26 +1,25 @@
 """The workgroup-per-cell kernel (mechanisms with 64 < nSpecie+1 <= 208) on
-MI355X: the batched form of the 200-species synthetic mechanism and of a
-100-species one (the two register-block sizes of the kernel), against the
-oracle, plus its determinism and its graph replay.
+MI355X: the 200-species synthetic mechanism and a 100-species one (the two
+register-block sizes of the kernel), in the drop-in and the persistent
+launch, against the oracle on contiguous, unfiltered seeded samples, plus
+its determinism and its graph replay.
 
-Tolerances are those of test_gpu_parity.py: species within rtol 1e-6 +
-absTol 1e-10, temperature within 1e-8 relative, identical status bits and
-step sequences, RR from the kernel's own final state.  The chemical time
-step is checked to 1e-4 relative instead of 1e-6: it scales with the
-error estimate to the power -1/4, and the estimate is a difference of stage
-vectors whose rounding follows the summation order, which here is blocked
-(32-column Jacobian assembly, 32-position triangular solves) instead of the
-reference's sequential order; for n = 201 that moves it by up to ~1e-5.
-For the same reason a cell near an accept/reject boundary can take a
-different number of attempts than the oracle.  Some cells of the synthetic
-mechanisms are that sensitive even to the last-bit differences between the
-device and host exp/log: the plain kernel, which keeps the reference's
-operation order, departs from the oracle on the same cells of the
-100-species set (cells 5, 9 and 10 of seed 11, one of them 3.6x the species
-tolerance).  Such a cell ends at a different, equally accurate state; the
-synthetic-mechanism checks therefore hold the tolerances above on every
-cell whose step sequence equals the oracle's, and bound the others at 10x
-the species tolerance and 1e-7 in temperature."""
+The bound is that of test_gpu_parity.py on every cell: species within
+rtol 1e-6 + absTol 1e-10 of the oracle, temperature within 1e-8 relative,
+identical status bits, identical step counts, deltaTChem within 1e-6
+relative.  The kernel sums in a blocked order (32-column Jacobian assembly,
+32-position triangular solves) instead of the reference's sequential one,
+and its exp/log round differently in the last bit.  A cell that misses the
+bound is accepted only if the oracle alone, on the CPU, shows the same
+sensitivity: the oracle is rerun from 1-ulp perturbations of that cell's
+initial state (T0 and each of its eight largest mass fractions, up and
+down: 18 runs); a differing step count must also occur among those runs,
+and each error may be at most 2x the largest change those perturbations
+cause in the oracle's own result (species error, T, deltaTChem).  Every
+such cell is printed with both figures.  (On the 100-species set, cells 5,
+9 and 10 of seed 11 are of this kind: their 1-ulp ensembles span 429-545,
+687-818 and 545-670 attempts and move the oracle's own final state by up
+to 3.9x the species tolerance.)"""
 import numpy as np
 import pytest
 
@@ -29,10 +28,13 @@ from gpufoam_amd.mechanism import Mechanism
 from gpufoam_amd.synthetic import syn_states, syn_texts
 from gpufoam_amd.workloads import make_states
 
-from test_gpu_parity import _check, _device_run
+from test_gpu_parity import _device_run
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
+
+RTOL_Y, ATOL_Y, RTOL_T, RTOL_DT = 1e-6, 1e-10, 1e-8, 1e-6
+ULP_FACTOR = 2.0
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -48,22 +50,55 @@ def _syn100():
     return Mechanism(t, r, "syn100")
 
 
-def _check_sensitive(m, got, ref, Y0, rho, deltaT, min_same=0.5):
+def _errors(nS, Ya, dta, Yb, dtb, rtol):
+    eY = (np.abs(Ya[:nS] - Yb[:nS]) / (rtol * np.abs(Yb[:nS]) + ATOL_Y)).max(axis=0)
+    eT = np.abs(Ya[nS] - Yb[nS]) / Yb[nS]
+    eD = np.abs(dta - dtb) / dtb
+    return eY, eT, eD
+
+
+def _check_conditioned(m, got, ref, Y0, dt0, c, ode, Treact=0.0, rho=None):
+    """Device result against the oracle, cell by cell (see the module
+    docstring); returns the number of cells that needed the oracle's own
+    1-ulp sensitivity to pass."""
+    from oracle_lib import Oracle
     Yg, dtg, _, stg, rrg = got
     Yo, dto, _, sto, _ = ref
     nS = m.n_species
-    errY = (np.abs(Yg[:nS] - Yo[:nS]) / (1e-6 * np.abs(Yo[:nS]) + 1e-10)).max(axis=0)
-    errT = np.abs(Yg[nS] - Yo[nS]) / Yo[nS]
-    same = stg[1] == sto[1]
-    print(f"identical step sequence in {same.mean():.3f} of cells; species error / "
-          f"tolerance max {errY[same].max():.3g} there, {errY.max():.3g} overall")
+    rtol = c["relTol"]
     assert (stg[0] == sto[0]).all()
     assert np.array_equal(Yg[nS + 1], Yo[nS + 1])
-    assert same.mean() >= min_same
-    assert errY[same].max() <= 1.0 and errT[same].max() <= 1e-8
-    assert np.allclose(dtg[same], dto[same], rtol=1e-4, atol=0)
-    assert errY.max() <= 10.0 and errT.max() <= 1e-7
-    assert np.allclose(rrg, rho * (Yg[:nS] - Y0[:nS]) / deltaT, rtol=1e-12, atol=1e-300)
+    if rho is not None:
+        assert np.allclose(rrg, rho * (Yg[:nS] - Y0[:nS]) / c["deltaT"], rtol=1e-12,
+                           atol=1e-300)
+    eY, eT, eD = _errors(nS, Yg, dtg, Yo, dto, rtol)
+    same = stg[1] == sto[1]
+    out = ~same | (eY > 1.0) | (eT > RTOL_T) | (eD > RTOL_DT)
+    print(f"{Y0.shape[1]} cells: identical step counts in {same.mean():.3f}; species "
+          f"err/tol max {eY.max():.3g}, T rel max {eT.max():.3g}, deltaTChem rel max "
+          f"{eD.max():.3g}; {out.sum()} cells beyond the bound")
+    for cell in np.nonzero(out)[0]:
+        Yc = np.ascontiguousarray(Y0[:, [cell]])
+        order = [int(i) for i in np.argsort(-Yc[:nS, 0])[:8] if Yc[i, 0] > 0]
+        rows = [nS] + order
+        P = np.repeat(Yc, 2 * len(rows), axis=1)
+        for k, (row, d) in enumerate((r, d) for r in rows for d in (np.inf, -np.inf)):
+            P[row, k] = np.nextafter(P[row, k], d)
+        rp = Oracle(m).solve_cells(ode, P, c["deltaT"], np.repeat(dt0[[cell]], P.shape[1]),
+                                   Treact=Treact)
+        sY, sT, sD = _errors(nS, rp[0], rp[1], Yo[:, [cell]], dto[[cell]], rtol)
+        spread = np.array([sY.max(), sT.max(), sD.max()])
+        moved = bool((rp[3][1] != sto[1][cell]).any())
+        print(f"  cell {cell}: attempts {stg[1][cell]} vs oracle {sto[1][cell]}; errors "
+              f"Y {eY[cell]:.3g} T {eT[cell]:.3g} dt {eD[cell]:.3g}; oracle's own 1-ulp "
+              f"spread Y {spread[0]:.3g} T {spread[1]:.3g} dt {spread[2]:.3g}, step count "
+              f"{'moves' if moved else 'fixed'}")
+        if not same[cell]:
+            assert moved, cell
+        assert eY[cell] <= max(1.0, ULP_FACTOR * spread[0]), cell
+        assert eT[cell] <= max(RTOL_T, ULP_FACTOR * spread[1]), cell
+        assert eD[cell] <= max(RTOL_DT, ULP_FACTOR * spread[2]), cell
+    return int(out.sum())
 
 
 def _oracle(m, Y0, dt0, c, rho, Treact=0.0):
@@ -75,24 +110,38 @@ def _oracle(m, Y0, dt0, c, rho, Treact=0.0):
 
 
 def test_block_kernel_syn200_matches_oracle():
-    """Twelve cells of the 200-species state law (n = 201: 13 x 7 register
-    blocks); two of them below Treact are skipped."""
+    """Cells 0..31 of the 200-species state law (n = 201: 13 x 7 register
+    blocks), unfiltered, in the drop-in and the persistent launch (which
+    agree bit for bit)."""
+    m, Y0, dt0, c = make_states("syn200", 32)
+    rho = np.linspace(0.4, 1.2, 32)
+    ode, ref = _oracle(m, Y0, dt0, c, rho)
+    print("case syn200 cells 0..31 plain", flush=True)
+    got = _device_run(m, Y0, dt0, c, ode, rho, batched=False)
+    _check_conditioned(m, got, ref, Y0, dt0, c, ode, rho=rho)
+    print("case syn200 cells 0..31 batched", flush=True)
+    bat = _device_run(m, Y0, dt0, c, ode, rho, batched=True)
+    for a, b in zip(got, bat):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+
+
+def test_block_kernel_syn200_treact():
+    """Cells at or below Treact are skipped (GFC_CELL_SKIPPED, deltaTChem =
+    deltaT) and the others integrated, cells 0..15."""
     m, Y0, dt0, c = make_states("syn200", 16)
-    keep = [0, 1, 3, 5, 6, 8, 9, 11, 12, 13, 14, 15]
-    Y0, dt0 = np.ascontiguousarray(Y0[:, keep]), dt0[keep]
     nS = m.n_species
     Treact = float(np.sort(Y0[nS])[1]) + 1e-9  # the two coolest cells stay put
-    rho = np.linspace(0.4, 1.2, len(keep))
+    rho = np.linspace(0.4, 1.2, 16)
     ode, ref = _oracle(m, Y0, dt0, c, rho, Treact)
-    print(f"case syn200 n={len(keep)} batched Treact={Treact}", flush=True)
+    print(f"case syn200 n=16 batched Treact={Treact}", flush=True)
     got = _device_run(m, Y0, dt0, c, ode, rho, batched=True, Treact=Treact)
     assert ((got[3][0] & 4) != 0).sum() == 2  # GFC_CELL_SKIPPED on two cells
-    _check(m, got, ref, tight_dt=True, Y0=Y0, rho=rho, deltaT=c["deltaT"], dt_rtol=1e-4)
+    _check_conditioned(m, got, ref, Y0, dt0, c, ode, Treact=Treact, rho=rho)
 
 
 def test_block_kernel_syn100_matches_oracle():
     """A 100-species / 600-reaction synthetic mechanism (n = 101: the
-    8 x 4 register-block instantiation)."""
+    8 x 4 register-block instantiation), cells 0..11 of seed 11."""
     m = _syn100()
     Y0, dt0 = syn_states(m, 12, seed=11)
     c = dict(solver="Rosenbrock34", relTol=1e-6, absTol=1e-10, deltaT=1e-6)
@@ -100,7 +149,7 @@ def test_block_kernel_syn100_matches_oracle():
     ode, ref = _oracle(m, Y0, dt0, c, rho)
     print("case syn100 n=12 batched", flush=True)
     got = _device_run(m, Y0, dt0, c, ode, rho, batched=True)
-    _check_sensitive(m, got, ref, Y0, rho, c["deltaT"])
+    _check_conditioned(m, got, ref, Y0, dt0, c, ode, rho=rho)
 
 
 def test_block_kernel_deterministic_and_graph_replay():
@@ -137,7 +186,7 @@ def test_block_kernel_other_solvers(solver):
     ode, ref = _oracle(m, Y0, dt0, c, rho)
     print(f"case syn100 n=8 {solver}", flush=True)
     got = _device_run(m, Y0, dt0, c, ode, rho, batched=True)
-    _check_sensitive(m, got, ref, Y0, rho, c["deltaT"])
+    _check_conditioned(m, got, ref, Y0, dt0, c, ode, rho=rho)
 
 
 def test_block_kernel_reports_max_steps():

@@ -321,25 +321,3 @@ def test_step_underflow_is_reported():
         Yn, dt, _, st, _ = ev.solve_host(c["deltaT"], Y0, dt0, batched=batched, lane=lane)
         assert np.array_equal(st[0], ref[3][0])
         assert np.array_equal(st[1], ref[3][1]) and np.array_equal(st[2], ref[3][2])
-
-
-def test_synthetic_200_species_matches_oracle():
-    """The 200-species / 1200-reaction synthetic mechanism (n = 201, beyond
-    the wavefront kernel's capacity: the plain form runs one lane per cell,
-    the batched form one workgroup per cell), on four cells of its state
-    law with short step sequences."""
-    from oracle_lib import Oracle
-    m, Y0, dt0, c = make_states("syn200", 16)
-    keep = [1, 5, 8, 11]
-    Y0, dt0 = np.ascontiguousarray(Y0[:, keep]), dt0[keep]
-    ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
-    rho = np.linspace(0.3, 1.3, len(keep))
-    ref = Oracle(m).solve_cells(ode, Y0, c["deltaT"], dt0, rho=rho, want_rr=True)
-    for batched in (False, True):
-        print(f"case syn200 n=4 batched={batched}", flush=True)
-        got = _device_run(m, Y0, dt0, c, ode, rho, batched=batched)
-        # the batched form runs the workgroup-per-cell kernel, whose blocked
-        # summation order moves the error estimate (and the next chemical
-        # time step) at the 1e-5 level; see test_gpu_block.py
-        _check(m, got, ref, tight_dt=True, Y0=Y0, rho=rho, deltaT=c["deltaT"],
-               dt_rtol=1e-4 if batched else 1e-6)

@@ -87,6 +87,10 @@ def lib() -> C.CDLL:
             "gfc_last_error": (C.c_char_p, []),
             "gfc_mechanism_from_foam": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(vp)]),
             "gfc_mechanism_destroy": (None, [vp]),
+            "gfc_chemkin_to_foam": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t,
+                                              C.POINTER(C.c_size_t), C.c_char_p, C.c_size_t,
+                                              C.POINTER(C.c_size_t)]),
+            "gfc_mechanism_from_chemkin": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(vp)]),
             "gfc_mechanism_view": (C.c_int, [vp, C.POINTER(MechView)]),
             "gfc_mechanism_species_name": (C.c_char_p, [vp, C.c_int]),
             "gfc_mechanism_reaction_equation": (C.c_char_p, [vp, C.c_int]),
@@ -147,6 +151,7 @@ def exported_symbols() -> list[str]:
         "gfc_adjoint_forward", "gfc_adjoint_backward", "gfc_adjoint_replay",
         "gfc_calculate", "gfc_tc", "gfc_reaction_rr", "gfc_specie_reaction_rr", "gfc_qdot",
         "gfc_chemistry_switches_from_foam", "gfc_solve_noalloc",
+        "gfc_chemkin_to_foam", "gfc_mechanism_from_chemkin",
     ]
 
 

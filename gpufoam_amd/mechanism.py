@@ -117,6 +117,13 @@ class Mechanism:
         return cls(t, r, name or os.path.basename(path))
 
     @classmethod
+    def from_chemkin(cls, chem_inp: str, therm_dat: str, name: str = "") -> "Mechanism":
+        """A mechanism from CHEMKIN-II texts (chem.inp, therm.dat), converted
+        as chemkinToFoam does (gfc_chemkin_to_foam) and compiled."""
+        thermo, reactions = chemkin_to_foam(chem_inp, therm_dat)
+        return cls(thermo, reactions, name)
+
+    @classmethod
     def load(cls, name: str) -> "Mechanism":
         """A bundled mechanism: 'h2' (10 species, 27 reactions), 'gri'
         (GRI-Mech 3.0, 53 species, 325 reactions) or 'syn200' (the seeded
@@ -134,6 +141,19 @@ class Mechanism:
                                 self.arrays["sThigh"], self.arrays["sTcommon"],
                                 self.raw_high, self.raw_low), \
             foam_reactions_text(self)
+
+
+def chemkin_to_foam(chem_inp: str, therm_dat: str) -> tuple[str, str]:
+    """(thermo, reactions) OpenFOAM dictionary texts of a CHEMKIN-II
+    mechanism (the chemkinToFoam conversion, gfc_chemkin_to_foam)."""
+    L = _capi.lib()
+    a, b = chem_inp.encode(), therm_dat.encode()
+    nt, nr = C.c_size_t(), C.c_size_t()
+    _capi.check(L.gfc_chemkin_to_foam(a, b, None, 0, C.byref(nt), None, 0, C.byref(nr)))
+    bt, br = C.create_string_buffer(nt.value), C.create_string_buffer(nr.value)
+    _capi.check(L.gfc_chemkin_to_foam(a, b, bt, nt.value, C.byref(nt), br, nr.value,
+                                      C.byref(nr)))
+    return bt.value.decode(), br.value.decode()
 
 
 def _num(x: float) -> str:

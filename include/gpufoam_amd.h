@@ -146,6 +146,23 @@ const char* gfc_last_error(void);
  *   optional Tlow/Thigh). */
 int gfc_mechanism_from_foam(const char* thermo_text, const char* reactions_text,
                             gfc_mechanism** out);
+/* CHEMKIN-II input, in place of OpenFOAM's chemkinToFoam
+ * (tutorials/h2/Allrun:10-12): chem.inp (ELEMENTS, SPECIES, optional
+ * THERMO, REACTIONS with CAL/MOLE, KCAL/MOLE, JOULES/MOLE, KJOULES/MOLE or
+ * KELVINS) and a NASA-7 thermo database (therm.dat).  Elementary, +M
+ * third-body and (+M)/(+specie) fall-off reactions with LOW/TROE/SRI,
+ * HIGH (chemically activated) and REV map to the OpenFOAM reaction types
+ * chemkinToFoam writes; rate constants are converted to kmol, m^3, s and
+ * Ta = Ea/R.  gfc_chemkin_to_foam writes the two OpenFOAM dictionaries
+ * (call with NULL buffers to get the lengths, NUL included);
+ * gfc_mechanism_from_chemkin compiles them as gfc_mechanism_from_foam
+ * does. */
+int gfc_chemkin_to_foam(const char* chem_inp, const char* therm_dat,
+                        char* thermo_out, size_t thermo_cap, size_t* thermo_len,
+                        char* reactions_out, size_t reactions_cap,
+                        size_t* reactions_len);
+int gfc_mechanism_from_chemkin(const char* chem_inp, const char* therm_dat,
+                               gfc_mechanism** out);
 void gfc_mechanism_destroy(gfc_mechanism* mech);
 int gfc_mechanism_view(const gfc_mechanism* mech, gfc_mech_view* out);
 /* Name of specie i (pointer valid while the mechanism lives). */

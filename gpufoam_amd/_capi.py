@@ -23,6 +23,9 @@ STATUS_NAMES = {
 }
 
 RATE_ARRHENIUS, RATE_THIRD_BODY, RATE_LINDEMANN, RATE_TROE = 0, 1, 2, 3
+RATE_SRI, RATE_LINDEMANN_CA, RATE_TROE_CA, RATE_SRI_CA = 4, 5, 6, 7
+RATE_LANDAU_TELLER, RATE_JANEV, RATE_POWER_SERIES = 8, 9, 10
+EXTRA_PARAMS = 9
 ROSENBROCK12, ROSENBROCK23, ROSENBROCK34 = 12, 23, 34
 SOLVER_NAMES = {"Rosenbrock12": 12, "Rosenbrock23": 23, "Rosenbrock34": 34}
 
@@ -67,6 +70,7 @@ class MechView(C.Structure):
         ("lhs_index", _ip), ("lhs_stoich", _dp), ("lhs_exp", _dp), ("lhs_exp_int", _ip),
         ("rhs_index", _ip), ("rhs_stoich", _dp), ("rhs_exp", _dp), ("rhs_exp_int", _ip),
         ("rY", _dp), ("rW", _dp), ("rTcommon", _dp), ("rhigh", _dp), ("rlow", _dp),
+        ("extra", _dp),
     ]
 
 

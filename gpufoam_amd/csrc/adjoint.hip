@@ -351,6 +351,11 @@ int gfc_adjoint_forward(gfc_evaluator* ev, double deltaT, int nCells, int ldY,
         setLastError("gfc_adjoint_forward: nCells exceeds the evaluator's maxCells");
         return GFC_ERR_CAPACITY;
     }
+    if (ev->dm.extended) {
+        setLastError("gfc_adjoint_forward: the discrete adjoint covers gpuFoam's rate types "
+                     "(Arrhenius, third body, Lindemann and Troe fall-off) only");
+        return GFC_ERR_UNSUPPORTED;
+    }
     AdjState* st = state(ev);
     st->valid = false;
     if (nCells == 0) {

@@ -191,7 +191,10 @@ static void choose_plan(gfc_evaluator* ev, int nCells, Plan& pl, int& cellsPerBl
     const int n = ev->nS + 1;
     pl = Plan{};
     cellsPerBlock = 1;
-    if (ev->cellsPerWave == 64) return;
+    // one lane per cell when asked for, and for mechanisms with rate types
+    // beyond gpuFoam's four (the wavefront and workgroup kernels evaluate
+    // those four from their packed records)
+    if (ev->cellsPerWave == 64 || ev->dmTeam.extended) return;
     int N16 = 0, N = 0;
     KernelFn t16 = team16_selected(ev, nCells, N16) ? team16_kernel_for(ev->dmTeam, N16)
                                                     : nullptr;
@@ -348,7 +351,7 @@ int gfc_evaluator_set_batched_shape(gfc_evaluator* ev, int cellsPerWave,
     }
     if (cellsPerWave == 4) {
         int N = 0;
-        if (!gfc::team16_kernel_for(ev->dmTeam, N)) {
+        if (!gfc::team16_kernel_for(ev->dmTeam, N) || ev->dmTeam.extended) {
             gfc::setLastError("gfc_evaluator_set_batched_shape: 4 cells per wavefront "
                               "needs nSpecie+1 <= 16, at most 16 third-body efficiency "
                               "sets and at most 3 species per reaction side");

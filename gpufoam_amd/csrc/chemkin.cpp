@@ -22,7 +22,8 @@
 //     1 unless listed; "(+M)" with LOW -> Lindemann fall-off, with TROE ->
 //     Troe fall-off, with SRI -> SRI fall-off; "(+SP)" -> efficiency 1 for
 //     SP and 0 for the others; HIGH (chemically activated) and REV (explicit
-//     reverse rate) map to their own OpenFOAM types;
+//     reverse rate) map to their own OpenFOAM types, and so do LT
+//     (Landau-Teller), JAN (Janev) and FIT1 (power series);
 //   * terms are written as in the input ("H + H + H2"), reactions named
 //     un-named-reaction-<index>.
 #include "mechanism.hpp"
@@ -133,7 +134,7 @@ struct Reaction {
     bool thirdBody = false;     // "+M"
     std::string fallOffBody;    // "" none, "M", or a specie
     double A = 0, beta = 0, Ea = 0;
-    std::vector<double> low, high, troe, sri, rev;
+    std::vector<double> low, high, troe, sri, rev, lt, jan, fit1;
     std::vector<std::pair<std::string, double>> eff;
     bool duplicate = false;
 };
@@ -309,10 +310,16 @@ void parseAuxiliary(const std::string& line, Reaction& r,
             r.sri = vals;
         } else if (K == "REV") {
             r.rev = vals;
+        } else if (K == "LT") {
+            r.lt = vals;
+        } else if (K == "JAN") {
+            r.jan = vals;
+        } else if (K == "FIT1") {
+            r.fit1 = vals;
         } else if (hasVals && species.count(key) && vals.size() == 1) {
             r.eff.emplace_back(key, vals[0]);
-        } else if (K == "PLOG" || K == "FORD" || K == "RORD" || K == "CHEB" || K == "LT" ||
-                   K == "RLT" || K == "JAN" || K == "FIT1" || K == "TDEP" || K == "EXCI" ||
+        } else if (K == "PLOG" || K == "FORD" || K == "RORD" || K == "CHEB" ||
+                   K == "RLT" || K == "TDEP" || K == "EXCI" ||
                    K == "MOME" || K == "XSMI" || K == "UNITS" || K == "HV") {
             fail("CHEMKIN keyword " + K + " is not supported (reaction '" + r.equation + "')",
                  GFC_ERR_UNSUPPORTED);
@@ -504,6 +511,32 @@ Translation translate(const std::string& chem, const std::string& thermoDb) {
             R << "        }\n        reverse\n        {\n";
             writeArrhenius(R, "            ", Aconv(r.rev[0], rorder), r.rev[1], Ta(r.rev[2]));
             R << "        }\n    }\n";
+            continue;
+        }
+        const int nT = !r.lt.empty() + !r.jan.empty() + !r.fit1.empty();
+        if (nT > 0) {
+            // temperature-dependent forms: LT (Landau-Teller), JAN (Janev),
+            // FIT1 (power series)
+            if (nT > 1 || r.thirdBody || !r.fallOffBody.empty())
+                fail("LT/JAN/FIT1 combine with neither each other nor a third body ('" +
+                     r.equation + "')", GFC_ERR_UNSUPPORTED);
+            const char* type = !r.lt.empty() ? "LandauTeller" : !r.jan.empty() ? "Janev"
+                                                                               : "powerSeries";
+            const auto& v = !r.lt.empty() ? r.lt : !r.jan.empty() ? r.jan : r.fit1;
+            const size_t want = !r.lt.empty() ? 2 : !r.jan.empty() ? 9 : 4;
+            if (v.size() != want) fail(std::string(type) + " needs " + std::to_string(want) +
+                                       " values in '" + r.equation + "'");
+            R << "        type " << pre << type << ";\n"
+              << "        reaction \"" << eqText << "\";\n";
+            writeArrhenius(R, "        ", Aconv(r.A, order), r.beta, Ta(r.Ea));
+            if (!r.lt.empty()) {
+                R << "        B " << num(v[0]) << ";\n        C " << num(v[1]) << ";\n";
+            } else {
+                R << "        " << (r.jan.empty() ? "coeffs " : "b ") << want << " (";
+                for (double x : v) R << " " << num(x);
+                R << " );\n";
+            }
+            R << "    }\n";
             continue;
         }
         if (r.fallOffBody.empty()) {

@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/gpufoam_amd.h"
 #include "constants.hpp"
 
 namespace gfc {
@@ -98,7 +99,17 @@ struct DeviceMech {
     const int* recI;
     int teamWidth; // widest reaction side (the record holds kTeamWidth)
     BlockTables blk;
+    // parameters of the rate types gpuFoam rejects, [nR*GFC_EXTRA_PARAMS],
+    // and whether the mechanism has any (only the one-lane and query
+    // kernels evaluate them)
+    const double* extra;
+    int extended;
 };
+
+// Rates that depend on a third-body concentration M.
+GFC_HD __forceinline__ bool uses_third_body(int type) {
+    return type >= GFC_RATE_THIRD_BODY && type <= GFC_RATE_SRI_CA;
+}
 
 constexpr int kSetsT = 16;
 
@@ -182,6 +193,72 @@ GFC_HD __forceinline__ double troe_F(const double* tr, double T,
     const double x3 = 1.0 + x2 * x2;
     const double x4 = logFcent / x3;
     return pow(10.0, x4);
+}
+
+// ------------------------------------- rate types gpuFoam rejects --
+// Published forms as OpenFOAM's rate classes take them
+// (makeMoreGpuChemistryModels.C:59-95 registers them on the CPU side only).
+
+// SRI fall-off function (Stewart, Larson & Golden 1989), x = a b c d e:
+// F = d [a exp(-b/T) + exp(-T/c)]^X T^e, X = 1/(1 + log10(max(Pr,small))^2)
+GFC_HD __forceinline__ double sri_F(const double* x, double T, double Pr) {
+    const double lp = log10(fmax(Pr, kSmall));
+    const double X = 1.0 / (1.0 + lp * lp);
+    return x[3] * pow(x[0] * exp(-x[1] / T) + exp(-T / x[2]), X) * pow(T, x[4]);
+}
+
+// Landau-Teller, Janev and power-series rates and dk/dT (ddT may be null):
+//   LT     k = A T^b exp(-Ta/T + B T^(-1/3) + C T^(-2/3))
+//   Janev  k = A T^b exp(-Ta/T) exp(sum_{n=0..8} b_n (ln T)^n)
+//   power  k = A T^b exp(-Ta/T + sum_{n=1..4} c_n T^(-n))
+GFC_HD __forceinline__ double tdep_rate(int type, const double* k, const double* x,
+                                        double T, double* ddT) {
+    double v = k[0];
+    if (fabs(k[1]) > kVSmall) v *= pow(T, k[1]);
+    double arg = 0.0, darg = 0.0;
+    if (fabs(k[2]) > kVSmall) {
+        arg -= k[2] / T;
+        darg += k[2] / (T * T);
+    }
+    if (type == GFC_RATE_LANDAU_TELLER) {
+        const double t13 = pow(T, 1.0 / 3.0);
+        if (fabs(x[0]) > kVSmall) {
+            arg += x[0] / t13;
+            darg -= x[0] / (3.0 * T * t13);
+        }
+        if (fabs(x[1]) > kVSmall) {
+            arg += x[1] / (t13 * t13);
+            darg -= 2.0 * x[1] / (3.0 * T * t13 * t13);
+        }
+    } else if (type == GFC_RATE_JANEV) {
+        const double lnT = log(T);
+        double Tn = 1.0, dsum = 0.0;
+        for (int n = 0; n < 9; ++n) {
+            arg += x[n] * Tn;
+            if (n + 1 < 9) dsum += (n + 1) * x[n + 1] * Tn;
+            Tn *= lnT;
+        }
+        darg += dsum / T;
+    } else {
+        double Tn = 1.0;
+        for (int n = 1; n <= 4; ++n) {
+            Tn *= T;
+            arg += x[n - 1] / Tn;
+            darg -= n * x[n - 1] / (Tn * T);
+        }
+    }
+    if (fabs(arg) > kVSmall) v *= exp(arg);
+    if (ddT) *ddT = v * (k[1] / T + darg);
+    return v;
+}
+
+// Blending function F(T, Pr) of a fall-off or chemically activated rate.
+GFC_HD __forceinline__ double blend_F(const DeviceMech& m, int r, double T, double Pr) {
+    const int t = m.type[r];
+    if (t == GFC_RATE_TROE || t == GFC_RATE_TROE_CA) return troe_F(m.troe + 4 * r, T, Pr);
+    if (t == GFC_RATE_SRI || t == GFC_RATE_SRI_CA)
+        return sri_F(m.extra + (long)GFC_EXTRA_PARAMS * r, T, Pr);
+    return 1.0;
 }
 
 // gpuSpecieExponent.H:21-36

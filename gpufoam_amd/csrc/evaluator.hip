@@ -74,6 +74,9 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     const auto rs = gather(v.rhs_stoich, kWidth), re = gather(v.rhs_exp, kWidth);
     const auto rY = gather(v.rY, 1), rW = gather(v.rW, 1), rTc = gather(v.rTcommon, 1);
     const auto rhigh = gather(v.rhigh, 7), rlow = gather(v.rlow, 7);
+    const auto extra = gather(v.extra, GFC_EXTRA_PARAMS);
+    int extended = 0;
+    for (int r = 0; r < nR; ++r) extended |= type[r] > GFC_RATE_TROE;
     const int nPass = (nR + 63) / 64;
     std::vector<double> effP((size_t)nPass * nS * 64, 0.0);
     for (int r = 0; r < nR; ++r) {
@@ -93,7 +96,7 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     std::vector<std::vector<double>> sets;
     std::vector<int> effSet(nR, -1);
     for (int r = 0; r < nR; ++r) {
-        if (type[r] == GFC_RATE_ARRHENIUS) continue;
+        if (!uses_third_body(type[r])) continue;
         std::vector<double> row(eff.begin() + (size_t)r * nS, eff.begin() + (size_t)(r + 1) * nS);
         int s = -1;
         for (size_t q = 0; q < sets.size(); ++q)
@@ -271,6 +274,7 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     add(tbNu.data(), tbNu.size());
     add(effB.data(), effB.size());
     add(effT.data(), effT.size());
+    add(extra.data(), extra.size());
     if (hipFail(hipMalloc(&block, blob.size()), "hipMalloc(mechanism)"))
         return GFC_ERR_HIP;
     if (hipFail(hipMemcpy(block, blob.data(), blob.size(), hipMemcpyHostToDevice),
@@ -304,6 +308,8 @@ static int upload_tables(const gfc_mech_view& v, const std::vector<int>& perm,
     at(d.blk.elPtr); at(d.blk.elRS); at(d.blk.elNu);
     at(d.blk.tbPtr); at(d.blk.tbR); at(d.blk.tbSet); at(d.blk.tbNu);
     at(d.blk.effB); at(d.blk.effT);
+    at(d.extra);
+    d.extended = extended;
     d.teamWidth = teamWidth;
     d.nPass = nPass;
     d.nSets = static_cast<int>(sets.size());
@@ -344,6 +350,7 @@ bool block_eligible(const gfc_mech_view& v) {
     for (int r = 0; r < v.n_reactions; ++r) {
         width = std::max({width, v.n_lhs[r], v.n_rhs[r]});
         nonArr += v.type[r] != GFC_RATE_ARRHENIUS;
+        if (v.type[r] > GFC_RATE_TROE) return false; // gpuFoam's four types only
     }
     return width <= kTeamWidth && nonArr <= kBlockMaxSets;
 }

@@ -97,6 +97,8 @@ GFC_HD inline double rate_k(const DeviceMech& m, int r, double T, SV c) {
     const double* k = m.k + 3 * r;
     const int t = m.type[r];
     if (t == GFC_RATE_ARRHENIUS) return arrhenius(k, T);
+    if (t > GFC_RATE_SRI_CA)
+        return tdep_rate(t, k, m.extra + (long)GFC_EXTRA_PARAMS * r, T, nullptr);
     const double* eff = m.eff + (long)r * m.nS;
     double M = 0;
     for (int i = 0; i < m.nS; ++i) M += eff[i] * c[i];
@@ -104,7 +106,8 @@ GFC_HD inline double rate_k(const DeviceMech& m, int r, double T, SV c) {
     const double k0 = arrhenius(k, T);
     const double kInf = arrhenius(m.kInf + 3 * r, T);
     const double Pr = k0 / kInf * M;
-    const double F = (t == GFC_RATE_TROE) ? troe_F(m.troe + 4 * r, T, Pr) : 1.0;
+    const double F = blend_F(m, r, T, Pr);
+    if (t >= GFC_RATE_LINDEMANN_CA) return k0 * (1.0 / (1.0 + Pr)) * F; // chemically activated
     return kInf * (Pr / (1.0 + Pr)) * F;
 }
 
@@ -209,6 +212,20 @@ GFC_HD __forceinline__ void jacobian(const DeviceMech& m, double p, SV y, LaneBu
             kf = M * Ak;
             ddT = M * dT;
             ddcScale = Ak;
+        } else if (t > GFC_RATE_SRI_CA) {
+            // Landau-Teller, Janev, power series: k(T) only
+            kf = tdep_rate(t, k, m.extra + (long)GFC_EXTRA_PARAMS * r, T, &ddT);
+        } else if (t >= GFC_RATE_LINDEMANN_CA) {
+            // chemically activated: the fall-off convention of gpuFoam
+            // applied to its leading factor k0 (no dPr/dT, dF/dT, dM/dc)
+            const double k0 = arrhenius(k, T);
+            const double kInf = arrhenius(m.kInf + 3 * r, T);
+            const double* eff = m.eff + (long)r * nS;
+            for (int i = 0; i < nS; ++i) M += eff[i] * c[i];
+            const double Pr = k0 / kInf * M;
+            const double temp = (1.0 / (1.0 + Pr)) * blend_F(m, r, T, Pr);
+            kf = k0 * temp;
+            ddT = (arrhenius(k, T) * (k[1] + k[2] / T) / T) * temp;
         } else {
             // gpuFallOffReactionRate.H:303-335 (kInf derivative only)
             const double k0 = arrhenius(k, T);
@@ -217,7 +234,7 @@ GFC_HD __forceinline__ void jacobian(const DeviceMech& m, double p, SV y, LaneBu
             const double* eff = m.eff + (long)r * nS;
             for (int i = 0; i < nS; ++i) M += eff[i] * c[i];
             const double Pr = k0 / kInf * M;
-            const double F = (t == GFC_RATE_TROE) ? troe_F(m.troe + 4 * r, T, Pr) : 1.0;
+            const double F = blend_F(m, r, T, Pr);
             const double temp = (Pr / (1.0 + Pr)) * F;
             kf = kInf * temp;
             ddT = (arrhenius(ki, T) * (ki[1] + ki[2] / T) / T) * temp;
@@ -274,8 +291,8 @@ GFC_HD __forceinline__ void jacobian(const DeviceMech& m, double p, SV y, LaneBu
         // jac_dCdT_contribution
         for (int i = 0; i < nl; ++i) b.Jat(m.li[o + i], nS) -= m.ls[o + i] * dwdT;
         for (int i = 0; i < nr; ++i) b.Jat(m.ri[o + i], nS) += m.rs[o + i] * dwdT;
-        // jac_dCdC_contribution
-        if (t != GFC_RATE_ARRHENIUS) {
+        // jac_dCdC_contribution (rates of a third-body concentration only)
+        if (uses_third_body(t)) {
             const double temp = (Cf - Cr / Kc);
             if (t == GFC_RATE_THIRD_BODY) {
                 const double* eff = m.eff + (long)r * nS;

@@ -296,34 +296,76 @@ gfc_mechanism* compileMechanism(const std::string& thermoText,
             if (pos != std::string::npos) typeName = typeName.substr(0, pos);
 
             double k[3] = {0, 0, 0}, kInf[3] = {0, 0, 0},
-                   troe[4] = {0, 0, 0, 0};
+                   troe[4] = {0, 0, 0, 0}, extra[GFC_EXTRA_PARAMS] = {};
             std::vector<double> eff(nS, 0.0);
             int type = -1, irreversible = 0;
-            if (typeName == "reversibleArrhenius" ||
-                typeName == "irreversibleArrhenius") {
-                type = GFC_RATE_ARRHENIUS;
-                irreversible = typeName[0] == 'i';
-                readArrhenius(rd, k);
-            } else if (typeName == "reversibleThirdBodyArrhenius") {
-                type = GFC_RATE_THIRD_BODY;
-                readArrhenius(rd, k);
-                readEfficiencies(rd, index, nS, eff.data());
-            } else if (typeName == "reversibleArrheniusLindemannFallOff" ||
-                       typeName == "reversibleArrheniusTroeFallOff") {
-                type = typeName.find("Troe") != std::string::npos
-                           ? GFC_RATE_TROE
-                           : GFC_RATE_LINDEMANN;
+            // gpuFoam's types (makeGpuReactionRate.H:115-134) and the
+            // OpenFOAM types it rejects (makeMoreGpuChemistryModels.C:59-95)
+            std::string base;
+            if (typeName.rfind("irreversible", 0) == 0) {
+                irreversible = 1;
+                base = typeName.substr(12);
+            } else if (typeName.rfind("reversible", 0) == 0) {
+                base = typeName.substr(10);
+            }
+            auto pressureDependent = [&](int t, const std::string& fn) {
+                type = t;
                 readArrhenius(rd.requireDict("k0"), k);
                 readArrhenius(rd.requireDict("kInf"), kInf);
                 readEfficiencies(rd.requireDict("thirdBodyEfficiencies"),
                                  index, nS, eff.data());
-                if (type == GFC_RATE_TROE) {
-                    const Dict& F = rd.requireDict("F");
+                const Dict& F = rd.requireDict("F");
+                if (fn == "Troe") {
                     troe[0] = F.scalar("alpha");
                     troe[1] = F.scalar("Tsss");
                     troe[2] = F.scalar("Ts");
                     troe[3] = F.scalar("Tss");
+                } else if (fn == "SRI") {
+                    extra[0] = F.scalar("a");
+                    extra[1] = F.scalar("b");
+                    extra[2] = F.scalar("c");
+                    extra[3] = F.scalarOr("d", 1.0);
+                    extra[4] = F.scalarOr("e", 0.0);
                 }
+            };
+            if (base == "Arrhenius") {
+                type = GFC_RATE_ARRHENIUS;
+                readArrhenius(rd, k);
+            } else if (base == "ThirdBodyArrhenius") {
+                type = GFC_RATE_THIRD_BODY;
+                readArrhenius(rd, k);
+                readEfficiencies(rd, index, nS, eff.data());
+            } else if (base == "ArrheniusLindemannFallOff") {
+                pressureDependent(GFC_RATE_LINDEMANN, "Lindemann");
+            } else if (base == "ArrheniusTroeFallOff") {
+                pressureDependent(GFC_RATE_TROE, "Troe");
+            } else if (base == "ArrheniusSRIFallOff") {
+                pressureDependent(GFC_RATE_SRI, "SRI");
+            } else if (base == "ArrheniusLindemannChemicallyActivated") {
+                pressureDependent(GFC_RATE_LINDEMANN_CA, "Lindemann");
+            } else if (base == "ArrheniusTroeChemicallyActivated") {
+                pressureDependent(GFC_RATE_TROE_CA, "Troe");
+            } else if (base == "ArrheniusSRIChemicallyActivated") {
+                pressureDependent(GFC_RATE_SRI_CA, "SRI");
+            } else if (base == "LandauTeller") {
+                type = GFC_RATE_LANDAU_TELLER;
+                readArrhenius(rd, k);
+                extra[0] = rd.scalar("B");
+                extra[1] = rd.scalar("C");
+            } else if (base == "Janev") {
+                type = GFC_RATE_JANEV;
+                readArrhenius(rd, k);
+                const auto b = rd.scalarList("b");
+                if (b.size() != 9)
+                    throw MechError{GFC_ERR_PARSE, "Janev rate needs 9 coefficients b"};
+                for (int q = 0; q < 9; ++q) extra[q] = b[q];
+            } else if (base == "powerSeries") {
+                type = GFC_RATE_POWER_SERIES;
+                readArrhenius(rd, k);
+                const auto c = rd.scalarList("coeffs");
+                if (c.size() != 4)
+                    throw MechError{GFC_ERR_PARSE, "powerSeries rate needs 4 coeffs"};
+                for (int q = 0; q < 4; ++q) extra[q] = c[q];
             } else {
                 throw MechError{GFC_ERR_UNSUPPORTED,
                                 "Invalid reaction type: " + rd.word("type")};
@@ -342,6 +384,7 @@ gfc_mechanism* compileMechanism(const std::string& thermoText,
             m->k.insert(m->k.end(), k, k + 3);
             m->kInf.insert(m->kInf.end(), kInf, kInf + 3);
             m->troe.insert(m->troe.end(), troe, troe + 4);
+            m->extra.insert(m->extra.end(), extra, extra + GFC_EXTRA_PARAMS);
             m->eff.insert(m->eff.end(), eff.begin(), eff.end());
             m->nLhs.push_back(static_cast<int>(lhs.size()));
             m->nRhs.push_back(static_cast<int>(rhs.size()));
@@ -438,6 +481,7 @@ gfc_mech_view gfc_mechanism::view() const {
     v.rTcommon = rTcommon.data();
     v.rhigh = rhigh.data();
     v.rlow = rlow.data();
+    v.extra = extra.data();
     return v;
 }
 

@@ -23,6 +23,7 @@ struct gfc_mechanism {
     std::vector<double> k, kInf, troe, eff, lhsStoich, lhsExp, rhsStoich,
         rhsExp;
     std::vector<double> rY, rW, rTcommon, rhigh, rlow;
+    std::vector<double> extra; // [nR*GFC_EXTRA_PARAMS]
 
     gfc_mech_view view() const;
 };

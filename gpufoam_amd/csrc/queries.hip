@@ -72,7 +72,10 @@ __device__ __forceinline__ void omega_fr(const DeviceMech& m, int r, double T,
     const double Tc = fmin(fmax(T, m.Tlow), m.Thigh);
     const int type = m.type[r];
     double kf = arrhenius(m.k + 3 * r, Tc);
-    if (type != GFC_RATE_ARRHENIUS) {
+    if (type > GFC_RATE_SRI_CA) {
+        // Landau-Teller, Janev, power series
+        kf = tdep_rate(type, m.k + 3 * r, m.extra + (long)GFC_EXTRA_PARAMS * r, Tc, nullptr);
+    } else if (type != GFC_RATE_ARRHENIUS) {
         // gpuThirdBodyEfficiencies.H:21-27, summed in specie order
         const double* eff = m.eff + (long)r * m.nS;
         double M = 0;
@@ -80,11 +83,13 @@ __device__ __forceinline__ void omega_fr(const DeviceMech& m, int r, double T,
         if (type == GFC_RATE_THIRD_BODY) {
             kf = M * kf; // gpuThirdBodyArrheniusReactionRate.H:39-44
         } else {
-            // gpuFallOffReactionRate.H:216-227
+            // gpuFallOffReactionRate.H:216-227; chemically activated
+            // rates blend from k0 instead: k0 / (1 + Pr) F
             const double kInf = arrhenius(m.kInf + 3 * r, Tc);
             const double Pr = kf / kInf * M;
-            const double F = (type == GFC_RATE_TROE) ? troe_F(m.troe + 4 * r, Tc, Pr) : 1.0;
-            kf = kInf * (Pr / (1.0 + Pr)) * F;
+            const double F = blend_F(m, r, Tc, Pr);
+            kf = (type >= GFC_RATE_LINDEMANN_CA) ? kf * (1.0 / (1.0 + Pr)) * F
+                                                 : kInf * (Pr / (1.0 + Pr)) * F;
         }
     }
     const double kr = m.irreversible[r] ? 0.0 : kf / reaction_Kc(m, r, Tc);

@@ -18,13 +18,23 @@ from . import _capi
 
 MECH_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mechanisms")
 
-_TYPE_NAMES = {
-    (_capi.RATE_ARRHENIUS, 0): "reversibleArrhenius",
-    (_capi.RATE_ARRHENIUS, 1): "irreversibleArrhenius",
-    (_capi.RATE_THIRD_BODY, 0): "reversibleThirdBodyArrhenius",
-    (_capi.RATE_LINDEMANN, 0): "reversibleArrheniusLindemannFallOff",
-    (_capi.RATE_TROE, 0): "reversibleArrheniusTroeFallOff",
+_BASE_NAMES = {
+    _capi.RATE_ARRHENIUS: "Arrhenius",
+    _capi.RATE_THIRD_BODY: "ThirdBodyArrhenius",
+    _capi.RATE_LINDEMANN: "ArrheniusLindemannFallOff",
+    _capi.RATE_TROE: "ArrheniusTroeFallOff",
+    _capi.RATE_SRI: "ArrheniusSRIFallOff",
+    _capi.RATE_LINDEMANN_CA: "ArrheniusLindemannChemicallyActivated",
+    _capi.RATE_TROE_CA: "ArrheniusTroeChemicallyActivated",
+    _capi.RATE_SRI_CA: "ArrheniusSRIChemicallyActivated",
+    _capi.RATE_LANDAU_TELLER: "LandauTeller",
+    _capi.RATE_JANEV: "Janev",
+    _capi.RATE_POWER_SERIES: "powerSeries",
 }
+_TYPE_NAMES = {(t, irr): ("irreversible" if irr else "reversible") + b
+               for t, b in _BASE_NAMES.items() for irr in (0, 1)}
+_PRESSURE_DEPENDENT = (_capi.RATE_LINDEMANN, _capi.RATE_TROE, _capi.RATE_SRI,
+                       _capi.RATE_LINDEMANN_CA, _capi.RATE_TROE_CA, _capi.RATE_SRI_CA)
 
 
 def _arr(ptr, n, dtype=np.float64):
@@ -64,6 +74,7 @@ class Mechanism:
         a["k"] = _arr(v.k, 3 * nR).reshape(nR, 3)
         a["kInf"] = _arr(v.kInf, 3 * nR).reshape(nR, 3)
         a["troe"] = _arr(v.troe, 4 * nR).reshape(nR, 4)
+        a["extra"] = _arr(v.extra, _capi.EXTRA_PARAMS * nR).reshape(nR, _capi.EXTRA_PARAMS)
         a["eff"] = _arr(v.eff, nS * nR).reshape(nR, nS)
         a["n_lhs"] = _arr(v.n_lhs, nR, np.int32)
         a["n_rhs"] = _arr(v.n_rhs, nR, np.int32)
@@ -199,15 +210,24 @@ def foam_reactions_text(m: "Mechanism", names=None) -> str:
         out += [f"    {nm}", "    {", f"        type            {tn};",
                 f"        reaction        \"{m.equations[r]}\";"]
         p = "        "
-        if t in (_capi.RATE_ARRHENIUS, _capi.RATE_THIRD_BODY):
+        x = a["extra"][r]
+        if t not in _PRESSURE_DEPENDENT:
             out += arr(p, a["k"][r])
             if t == _capi.RATE_THIRD_BODY:
                 out += effs(p, r)
+            elif t == _capi.RATE_LANDAU_TELLER:
+                out += [f"{p}B               {_num(x[0])};", f"{p}C               {_num(x[1])};"]
+            elif t == _capi.RATE_JANEV:
+                out += [f"{p}b               9 ( " + " ".join(_num(v) for v in x[:9]) + " );"]
+            elif t == _capi.RATE_POWER_SERIES:
+                out += [f"{p}coeffs          4 ( " + " ".join(_num(v) for v in x[:4]) + " );"]
         else:
             out += [p + "k0", p + "{"] + arr(p + "    ", a["k"][r]) + [p + "}"]
             out += [p + "kInf", p + "{"] + arr(p + "    ", a["kInf"][r]) + [p + "}"]
             out += [p + "F", p + "{"]
-            if t == _capi.RATE_TROE:
+            if t in (_capi.RATE_SRI, _capi.RATE_SRI_CA):
+                out += [f"{p}    {k}               {_num(v)};" for k, v in zip("abcde", x[:5])]
+            if t in (_capi.RATE_TROE, _capi.RATE_TROE_CA):
                 tr = a["troe"][r]
                 out += [f"{p}    alpha           {_num(tr[0])};",
                         f"{p}    Tsss            {_num(tr[1])};",

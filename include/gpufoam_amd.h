@@ -53,8 +53,20 @@ typedef enum gfc_rate_type {
     GFC_RATE_ARRHENIUS = 0,          /* gpuArrheniusReactionRate */
     GFC_RATE_THIRD_BODY = 1,         /* gpuThirdBodyArrheniusReactionRate */
     GFC_RATE_LINDEMANN = 2,          /* FallOff<Arrhenius, Lindemann> */
-    GFC_RATE_TROE = 3                /* FallOff<Arrhenius, Troe> */
+    GFC_RATE_TROE = 3,               /* FallOff<Arrhenius, Troe> */
+    /* Rate types OpenFOAM supports and gpuFoam rejects
+     * (makeMoreGpuChemistryModels.C:59-95); parameters beyond A, beta, Ta
+     * are in gfc_mech_view.extra. */
+    GFC_RATE_SRI = 4,                /* FallOff<Arrhenius, SRI>: extra a b c d e */
+    GFC_RATE_LINDEMANN_CA = 5,       /* ChemicallyActivated<Arrhenius, Lindemann> */
+    GFC_RATE_TROE_CA = 6,            /* ChemicallyActivated<Arrhenius, Troe> */
+    GFC_RATE_SRI_CA = 7,             /* ChemicallyActivated<Arrhenius, SRI>: extra a b c d e */
+    GFC_RATE_LANDAU_TELLER = 8,      /* LandauTeller: extra B C */
+    GFC_RATE_JANEV = 9,              /* Janev: extra b0 .. b8 */
+    GFC_RATE_POWER_SERIES = 10       /* powerSeries: extra c1 .. c4 */
 } gfc_rate_type;
+
+#define GFC_EXTRA_PARAMS 9 /* extra rate parameters per reaction */
 
 typedef enum gfc_solver {
     GFC_ROSENBROCK12 = 12,
@@ -129,6 +141,9 @@ typedef struct gfc_mech_view {
     const double* rTcommon;
     const double* rhigh; /* [nr*7] */
     const double* rlow;  /* [nr*7] */
+    /* [nr*GFC_EXTRA_PARAMS] parameters of the extended rate types (zero
+     * for gpuFoam's four) */
+    const double* extra;
 } gfc_mech_view;
 
 typedef struct gfc_mechanism gfc_mechanism;

@@ -163,10 +163,106 @@ static double troe_F(const double* tr, double T, double Pr) {
     return pow(10.0, x4);
 }
 
+/* rates that depend on a third-body concentration M (these carry the
+ * reference's dense concentration-derivative row, zero for fall-off) */
+static inline int uses_third_body(int type) {
+    return type >= GFC_RATE_THIRD_BODY && type <= GFC_RATE_SRI_CA;
+}
+
+/* ---- rate types gpuFoam rejects (makeMoreGpuChemistryModels.C:59-95);
+ * restated from their published forms, as OpenFOAM's rate classes take
+ * them (no gpuFoam counterpart, so no reference vectors: pinned by closed
+ * forms and finite differences, tests/test_extended_rates.py). */
+
+/* SRI fall-off function (Stewart, Larson & Golden 1989):
+ * F = d [a exp(-b/T) + exp(-T/c)]^X T^e, X = 1/(1 + log10(max(Pr,small))^2) */
+static double sri_F(const double* x, double T, double Pr) {
+    const double lp = log10(fmax(Pr, SMALL_));
+    const double X = 1.0 / (1.0 + lp * lp);
+    return x[3] * pow(x[0] * exp(-x[1] / T) + exp(-T / x[2]), X) * pow(T, x[4]);
+}
+
+/* fall-off / chemically-activated blending function F(T, Pr) */
+static double blend_F(const gfc_mech_view* m, int r, double T, double Pr) {
+    const int t = m->type[r];
+    if (t == GFC_RATE_TROE || t == GFC_RATE_TROE_CA) return troe_F(m->troe + 4 * r, T, Pr);
+    if (t == GFC_RATE_SRI || t == GFC_RATE_SRI_CA)
+        return sri_F(m->extra + (long)GFC_EXTRA_PARAMS * r, T, Pr);
+    return 1.0;
+}
+
+/* Landau-Teller, Janev and power-series rates and their dk/dT:
+ *   LT     k = A T^b exp(-Ta/T + B T^(-1/3) + C T^(-2/3))
+ *   Janev  k = A T^b exp(-Ta/T) exp(sum_{n=0..8} b_n (ln T)^n)
+ *   power  k = A T^b exp(-Ta/T + sum_{n=1..4} c_n T^(-n))
+ * each factor skipped when its parameter is below vSmall in magnitude, as
+ * Arrhenius does. */
+static double tdep_rate(const gfc_mech_view* m, int r, double T, double* ddT) {
+    const double* k = m->k + 3 * r;
+    const double* x = m->extra + (long)GFC_EXTRA_PARAMS * r;
+    double v = k[0];
+    if (fabs(k[1]) > VSMALL_) v *= pow(T, k[1]);
+    double arg = 0.0, darg = 0.0; /* exponent and its T derivative */
+    if (fabs(k[2]) > VSMALL_) {
+        arg -= k[2] / T;
+        darg += k[2] / (T * T);
+    }
+    switch (m->type[r]) {
+    case GFC_RATE_LANDAU_TELLER: {
+        const double t13 = pow(T, 1.0 / 3.0);
+        if (fabs(x[0]) > VSMALL_) {
+            arg += x[0] / t13;
+            darg -= x[0] / (3.0 * T * t13);
+        }
+        if (fabs(x[1]) > VSMALL_) {
+            arg += x[1] / (t13 * t13);
+            darg -= 2.0 * x[1] / (3.0 * T * t13 * t13);
+        }
+        break;
+    }
+    case GFC_RATE_JANEV: {
+        const double lnT = log(T);
+        double Tn = 1.0, dsum = 0.0;
+        for (int n = 0; n < 9; ++n) {
+            arg += x[n] * Tn;
+            if (n + 1 < 9) dsum += (n + 1) * x[n + 1] * Tn;
+            Tn *= lnT;
+        }
+        darg += dsum / T;
+        break;
+    }
+    default: { /* power series */
+        double Tn = 1.0;
+        for (int n = 1; n <= 4; ++n) {
+            Tn *= T;
+            arg += x[n - 1] / Tn;
+            darg -= n * x[n - 1] / (Tn * T);
+        }
+        break;
+    }
+    }
+    if (fabs(arg) > VSMALL_) v *= exp(arg);
+    if (ddT) *ddT = v * (k[1] / T + darg);
+    return v;
+}
+
 /* gpuReactionRate::operator() dispatch (gpuReactionRate.H:135-173) */
 static double rate_k(const gfc_mech_view* m, int r, double T, const double* c) {
     const double* k = m->k + 3 * r;
     switch (m->type[r]) {
+    case GFC_RATE_LANDAU_TELLER:
+    case GFC_RATE_JANEV:
+    case GFC_RATE_POWER_SERIES:
+        return tdep_rate(m, r, T, NULL);
+    case GFC_RATE_LINDEMANN_CA:
+    case GFC_RATE_TROE_CA:
+    case GFC_RATE_SRI_CA: {
+        /* chemically activated: k = k0 / (1 + Pr) F */
+        const double k0 = arrhenius(k, T);
+        const double kInf = arrhenius(m->kInf + 3 * r, T);
+        const double Pr = k0 / kInf * third_body_M(m, r, c);
+        return k0 * (1.0 / (1.0 + Pr)) * blend_F(m, r, T, Pr);
+    }
     case GFC_RATE_ARRHENIUS: return arrhenius(k, T);
     case GFC_RATE_THIRD_BODY:
         /* gpuThirdBodyArrheniusReactionRate.H:39-44 */
@@ -177,9 +273,7 @@ static double rate_k(const gfc_mech_view* m, int r, double T, const double* c) {
         const double kInf = arrhenius(m->kInf + 3 * r, T);
         const double M = third_body_M(m, r, c);
         const double Pr = k0 / kInf * M;
-        const double F = (m->type[r] == GFC_RATE_TROE)
-                             ? troe_F(m->troe + 4 * r, T, Pr)
-                             : 1.0;
+        const double F = blend_F(m, r, T, Pr);
         return kInf * (Pr / (1.0 + Pr)) * F;
     }
     }
@@ -209,6 +303,29 @@ static void rate_everything(const gfc_mech_view* m, int r, double T,
         out[1] = M * dT;
         return;
     }
+    case GFC_RATE_LANDAU_TELLER:
+    case GFC_RATE_JANEV:
+    case GFC_RATE_POWER_SERIES: {
+        double d;
+        out[0] = tdep_rate(m, r, T, &d);
+        out[1] = d;
+        return;
+    }
+    case GFC_RATE_LINDEMANN_CA:
+    case GFC_RATE_TROE_CA:
+    case GFC_RATE_SRI_CA: {
+        /* gpuFoam's fall-off convention carried over: only the leading
+         * Arrhenius factor (k0 here) is differentiated in T, and there is
+         * no concentration derivative */
+        const double k0 = arrhenius(k, T);
+        const double kInf = arrhenius(m->kInf + 3 * r, T);
+        const double Pr = k0 / kInf * third_body_M(m, r, c);
+        const double temp = (1.0 / (1.0 + Pr)) * blend_F(m, r, T, Pr);
+        out[0] = k0 * temp;
+        out[1] = arrhenius_ddT(k, T) * temp;
+        for (int i = 0; i < nS; ++i) ddc[i] = 0.0;
+        return;
+    }
     default: {
         /* gpuFallOffReactionRate.H:303-335: the Jacobian keeps only the
          * kInf temperature derivative and no concentration derivative. */
@@ -216,9 +333,7 @@ static void rate_everything(const gfc_mech_view* m, int r, double T,
         const double kInf = arrhenius(m->kInf + 3 * r, T);
         const double M = third_body_M(m, r, c);
         const double Pr = k0 / kInf * M;
-        const double F = (m->type[r] == GFC_RATE_TROE)
-                             ? troe_F(m->troe + 4 * r, T, Pr)
-                             : 1.0;
+        const double F = blend_F(m, r, T, Pr);
         const double temp = (Pr / (1.0 + Pr)) * F;
         const double kk = kInf * temp;
         const double dkInfdT = arrhenius_ddT(m->kInf + 3 * r, T);
@@ -343,7 +458,7 @@ void orc_reaction_parameters(const gfc_mech_view* m, int r, double p,
         side_derivs(m, r, 1, c, pw, dCr);
         for (int j = 0; j < m->n_rhs[r]; ++j) dCr[j] *= kr;
     }
-    if (m->type[r] != GFC_RATE_ARRHENIUS) {
+    if (uses_third_body(m->type[r])) {
         const double temp = (Cf - Cr / Kc);
         for (int j = 0; j < nS; ++j) ddc[j] *= temp;
     }
@@ -498,7 +613,7 @@ void orc_jacobian(const gfc_mech_view* m, const double* y, double* dfdx,
         for (int i = 0; i < nr; ++i)
             J_(m->rhs_index[b + i], nS) += m->rhs_stoich[b + i] * o[1];
         /* jac_dCdC_contribution, :614-648 */
-        if (m->type[r] != GFC_RATE_ARRHENIUS) {
+        if (uses_third_body(m->type[r])) {
             for (int i = 0; i < nl; ++i) {
                 double* row = &J_(m->lhs_index[b + i], 0);
                 const double sl = m->lhs_stoich[b + i];

@@ -54,7 +54,7 @@ def cpu_baseline(m, Y0, dt0, c, config: str, sample: int, threads: int,
     first `sample` cells of the benchmarked workload.  The CPU result on
     the sample is also the yardstick of the GPU result's error."""
     sys.path.insert(0, os.path.join(HERE, "tests"))
-    from gpufoam_amd import ode_inputs
+    from gpufoam_amd import _capi, ode_inputs
     import oracle_lib
     Ys = np.ascontiguousarray(Y0[:, :sample])
     dts = np.ascontiguousarray(dt0[:sample])
@@ -87,7 +87,7 @@ def adjoint_cost(n_cells: int, config: str = "h2_adjoint", deltaT: float = 0.0) 
     sum(T + 1e3 Y_OH), or sum(T) for mechanisms without OH.  deltaT > 0
     replaces the workload's step."""
     import torch
-    from gpufoam_amd import ode_inputs
+    from gpufoam_amd import _capi, ode_inputs
     from gpufoam_amd.evaluator import Evaluator
     from gpufoam_amd.workloads import make_states
     m, Y0, dt0, c = make_states(config, n_cells)
@@ -145,7 +145,7 @@ def batching_block(n_cells: int, graph: bool) -> dict:
     hipGraph) against the drop-in launch of the same kernel, one CFD step
     each on the same cells, and the per-cell attempt histogram."""
     import torch
-    from gpufoam_amd import ode_inputs
+    from gpufoam_amd import _capi, ode_inputs
     from gpufoam_amd.evaluator import Evaluator
     from gpufoam_amd.workloads import make_states
     m, Y0, dt0, c = make_states("gri_batching", n_cells)
@@ -204,6 +204,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--factorization", choices=["column", "panel", "mfma"], default=None,
+                    help="LU variant of the workgroup-per-cell kernel (64 < nSpecie+1 <= 208)")
     ap.add_argument("--adjoint-cells", type=int, default=65536,
                     help="cells of the differentiation workload for the adjoint cost (0: skip)")
     ap.add_argument("--adjoint-config", default="h2_adjoint",
@@ -234,7 +236,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": nb, "copy_bytes_written": nb, "calls": 3}))
         return
-    from gpufoam_amd import ode_inputs
+    from gpufoam_amd import _capi, ode_inputs
     from gpufoam_amd.evaluator import Evaluator
     from gpufoam_amd.workloads import CONFIGS, make_states
     from gpufoam_amd.workmodel import attempt_flops, work_flops
@@ -251,6 +253,9 @@ def main():
     dmin = torch.zeros(1, dtype=torch.float64, device=dev)
     stats = torch.zeros((3, n), dtype=torch.int32, device=dev)
     ev = Evaluator(m, ode, 0.0, max_cells=n)
+    if args.factorization:
+        ev.set_factorization({"column": _capi.LU_COLUMN, "panel": _capi.LU_PANEL,
+                              "mfma": _capi.LU_PANEL_MFMA}[args.factorization])
     graph = not args.no_graph
     tuned = None
     if args.autotune:
@@ -363,7 +368,8 @@ def main():
         "config": {"workload": args.config, "mechanism": cfg["mech"],
                    "species": m.n_species, "reactions": m.n_reactions, "cells": n,
                    "solver": c["solver"], "relTol": c["relTol"], "absTol": c["absTol"],
-                   "deltaT": c["deltaT"], "graph": graph},
+                   "deltaT": c["deltaT"], "graph": graph,
+                   "factorization": args.factorization or "default"},
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": traffic,

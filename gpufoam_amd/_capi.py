@@ -37,6 +37,8 @@ HOST_POINTERS = 0x1
 BATCHED = 0x2
 GRAPH = 0x4
 LANE_PER_CELL = 0x8
+# factorisation of the workgroup-per-cell kernel
+LU_COLUMN, LU_PANEL, LU_PANEL_MFMA = 0, 1, 2
 
 
 class GfcError(RuntimeError):
@@ -111,6 +113,8 @@ def lib() -> C.CDLL:
                                     vp, vp, vp, vp, vp, vp, vp, vp, C.c_uint]),
             "gfc_evaluator_set_batched_shape": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
             "gfc_evaluator_get_batched_shape": (C.c_int, [vp, _ip, _ip, _ip]),
+            "gfc_evaluator_set_factorization": (C.c_int, [vp, C.c_int]),
+            "gfc_evaluator_get_factorization": (C.c_int, [vp, _ip]),
             "gfc_evaluator_param_count": (C.c_int, [vp]),
             "gfc_evaluator_get_rate_parameters": (C.c_int, [vp, _dp]),
             "gfc_evaluator_set_rate_parameters": (C.c_int, [vp, _dp]),
@@ -149,7 +153,8 @@ def exported_symbols() -> list[str]:
         "gfc_mechanism_species_index", "gfc_ode_inputs_from_foam",
         "gfc_solver_from_name", "gfc_ode_inputs_default", "gfc_evaluator_create",
         "gfc_evaluator_destroy", "gfc_solve", "gfc_evaluator_set_batched_shape",
-        "gfc_evaluator_get_batched_shape", "gfc_evaluator_param_count",
+        "gfc_evaluator_get_batched_shape", "gfc_evaluator_set_factorization",
+        "gfc_evaluator_get_factorization", "gfc_evaluator_param_count",
         "gfc_evaluator_get_rate_parameters", "gfc_evaluator_set_rate_parameters",
         "gfc_adjoint_set_checkpoint_interval", "gfc_adjoint_checkpoint_interval",
         "gfc_adjoint_forward", "gfc_adjoint_backward", "gfc_adjoint_replay",

@@ -25,7 +25,7 @@ template <int N>
 __global__ void team_solve_kernel(SolveArgs A, int* queue);
 template <int N>
 __global__ void team16_solve_kernel(SolveArgs A, int* queue);
-template <int NA, int NC>
+template <int NA, int NC, int LUV>
 __global__ void block_solve_kernel(SolveArgs A, int* queue, double* scratch, long perWG);
 __global__ void lane_solve_kernel(SolveArgs A, double* scratch, long per);
 
@@ -56,9 +56,14 @@ using BlockFn = void (*)(SolveArgs, int*, double*, long);
 constexpr int kBlockThreads = 512;
 constexpr int kLanesPerWave = 64;
 
-BlockFn block_kernel_for(int NA) {
-    if (NA == 8) return block_solve_kernel<8, 4>;
-    if (NA == 13) return block_solve_kernel<13, 7>;
+BlockFn block_kernel_for(int NA, int lu) {
+    static const BlockFn k8[3] = {block_solve_kernel<8, 4, 0>, block_solve_kernel<8, 4, 1>,
+                                  block_solve_kernel<8, 4, 2>};
+    static const BlockFn k13[3] = {block_solve_kernel<13, 7, 0>, block_solve_kernel<13, 7, 1>,
+                                   block_solve_kernel<13, 7, 2>};
+    if (lu < 0 || lu > 2) return nullptr;
+    if (NA == 8) return k8[lu];
+    if (NA == 13) return k13[lu];
     return nullptr;
 }
 
@@ -114,11 +119,16 @@ int batched_init(gfc_evaluator* ev) {
     if (block_eligible(ev->mech.view()) && ev->dmTeam.nSets <= kBlockMaxSets) {
         const int NA = n <= 128 ? 8 : 13;
         const int NC = NA == 8 ? 4 : 7;
-        int occ = 0;
-        if (hipFail(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &occ, (const void*)block_kernel_for(NA), kBlockThreads, 0),
-                    "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
-            return GFC_ERR_HIP;
+        // resident workgroups: the smallest over the factorisation variants
+        int occ = 1 << 30;
+        for (int lu = 0; lu < 3; ++lu) {
+            int o = 0;
+            if (hipFail(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                            &o, (const void*)block_kernel_for(NA, lu), kBlockThreads, 0),
+                        "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
+                return GFC_ERR_HIP;
+            occ = o < occ ? o : occ;
+        }
         if (occ > 0) {
             b->blkNA = NA;
             b->blkGrid = b->numCUs * occ;
@@ -213,9 +223,9 @@ static void choose_plan(gfc_evaluator* ev, int nCells, Plan& pl, int& cellsPerBl
         pl.grid = b->numCUs * perCU;
         pl.key = N;
     } else if (b->blkNA > 0) {
-        pl.blk = block_kernel_for(b->blkNA);
+        pl.blk = block_kernel_for(b->blkNA, ev->factorization);
         pl.grid = b->blkGrid;
-        pl.key = 1000 + b->blkNA;
+        pl.key = 1000 + 10 * b->blkNA + ev->factorization;
     }
 }
 
@@ -369,6 +379,20 @@ int gfc_evaluator_get_batched_shape(const gfc_evaluator* ev, int* cellsPerWave,
     if (cellsPerWave) *cellsPerWave = ev->cellsPerWave;
     if (wavesPerBlock) *wavesPerBlock = ev->wavesPerBlock;
     if (chunk) *chunk = ev->chunk;
+    return GFC_OK;
+}
+int gfc_evaluator_set_factorization(gfc_evaluator* ev, int variant) {
+    if (!ev || variant < GFC_LU_COLUMN || variant > GFC_LU_PANEL_MFMA) {
+        gfc::setLastError("gfc_evaluator_set_factorization: variant must be GFC_LU_COLUMN, "
+                          "GFC_LU_PANEL or GFC_LU_PANEL_MFMA");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    ev->factorization = variant;
+    return GFC_OK;
+}
+int gfc_evaluator_get_factorization(const gfc_evaluator* ev, int* variant) {
+    if (!ev || !variant) return GFC_ERR_INVALID_ARGUMENT;
+    *variant = ev->factorization;
     return GFC_OK;
 }
 }

@@ -65,6 +65,7 @@ struct BlockLDS {
     unsigned long long vmax[NV];
     int perm[NV], pos[NV], poison[NV];
     int piv[2];
+    int ppiv[2][4]; // pivot rows of the current / next 4-column panel
     int cell;
 };
 
@@ -130,6 +131,14 @@ __device__ __forceinline__ void row_out(const double (&A)[NA][NC], int aP, int b
     if constexpr (b < NC) {
         if (b >= bK) rb[b] = row_pick<NA, NC, b>(A, aP);
         row_out<NA, NC, b + 1>(A, aP, bK, rb);
+    }
+}
+
+template <int NA, int NC, int b = 0>
+__device__ __forceinline__ void row_vals(const double (&A)[NA][NC], int aP, double (&out)[NC]) {
+    if constexpr (b < NC) {
+        out[b] = row_pick<NA, NC, b>(A, aP);
+        row_vals<NA, NC, b + 1>(A, aP, out);
     }
 }
 
@@ -638,12 +647,275 @@ __device__ __forceinline__ void lu_cols(double (&A)[NA][NC], int n, BlockLDS<NA,
     }
 }
 
-template <int NA, int NC, int B = 0>
+// Memory ordering of LDS among the lanes of one wavefront.
+__device__ __forceinline__ void wave_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// A[i][j] -= sum_p L[i][p] U[p][j] for register columns b >= B0 (and
+// b == bK only where liveK), one pivot after the other, so that every entry
+// sees its four fmas in pivot order and only one pivot row's values are
+// held at a time.
+template <int NA, int NC, int bK, int B0>
+__device__ __forceinline__ void rank4_valu(double (&A)[NA][NC], const double* Lb,
+                                           const double* Ub, int np, bool liveK,
+                                           const Coord& q) {
+    constexpr int NV = kPR * NA;
+    if constexpr (B0 < NC) {
+        for (int p = 0; p < np; ++p) {
+            double u[NC];
+#pragma unroll
+            for (int b = B0; b < NC; ++b) u[b] = Ub[(p * 4 + q.g) * NC + b];
+            const double* lp = Lb + p * NV + q.tr;
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                const double la = lp[kPR * a];
+#pragma unroll
+                for (int b = B0; b < NC; ++b) {
+                    if (b == bK) A[a][b] = liveK ? fma(-la, u[b], A[a][b]) : A[a][b];
+                    else A[a][b] = fma(-la, u[b], A[a][b]);
+                }
+            }
+        }
+    }
+}
+
+// Columns 32 bK .. 32 bK + 31 as eight 4-column panels.  Panel columns
+// k0 .. k0+3 sit in register column bK of the four thread columns of one
+// wavefront wP = (k0 % 32) / 4, so that wavefront factorises the panel
+// alone, with wavefront-level exchanges; one workgroup barrier publishes
+// the panel's pivots and multipliers, and every thread then computes the
+// four pivot rows' trailing entries U (in its thread column, by the lane
+// holding each pivot row) and applies the rank-4 update
+// A[i][j] -= sum_p L[i][p] U[p][j].  The arithmetic of every entry is the
+// per-column algorithm's (the same fma sequence in the same order), so with
+// vector FMAs the factors are bit-identical to lu_cols'.  With MFMA the
+// rank-4 update is one v_mfma_f64_16x16x4f64 per 16 x 16 tile: a
+// wavefront's entries A[a][b0..b0+3] are the accumulator tile
+// D[4r + g][tr] of the transposed update D -= U^T L^T (lane l holds
+// D[4r + l/16][l%16] in register r), and each lane loads one U and one L
+// value from LDS as the tile's operands.
+// Scratch in L.R (free during the factorisation): panel multipliers
+// Lb[par][p][row] and per-wavefront pivot-row values Ub[w][p][g][b].
+template <int NA, int NC, int bK, bool MFMA>
+__device__ __forceinline__ void lu_panels(double (&A)[NA][NC], int n, BlockLDS<NA, NC>& L,
+                                          const Coord& q0, unsigned& pivoted, int& par) {
+    const Coord q = opaque(q0);
+    constexpr int NV = BlockLDS<NA, NC>::NV;
+    static_assert(2 * 4 * NV + kBW * 16 * NC <= BlockLDS<NA, NC>::RSZ, "panel scratch");
+    for (int wP = 0; wP < kBW; ++wP) {
+        const int k0 = kPC * bK + 4 * wP;
+        if (k0 >= n) break;
+        const int np = min(4, n - k0);
+        double* Lb = L.R + par * (4 * NV);
+        double* Ub = L.R + 2 * 4 * NV + q.w * (16 * NC);
+        // ---- the panel, by wavefront wP
+        if (q.w == wP) {
+            for (int p = 0; p < np; ++p) {
+                const int k = k0 + p;
+                if (q.g == p) {
+                    double keyv[NA];
+                    double kmax = -2.0;
+#pragma unroll
+                    for (int a = 0; a < NA; ++a) {
+                        const int i = q.tr + kPR * a;
+                        const bool cand = i < n && !((pivoted >> a) & 1u);
+                        double key = L.vv[i] * fabs(A[a][bK]);
+                        if (!(key >= 0.0)) key = -1.0;
+                        keyv[a] = cand ? key : -2.0;
+                        kmax = fmax(kmax, keyv[a]);
+                    }
+                    kmax = fmax(kmax, dpp<0x121>(kmax, kmax));
+                    kmax = fmax(kmax, dpp<0x122>(kmax, kmax));
+                    kmax = fmax(kmax, dpp<0x124>(kmax, kmax));
+                    kmax = fmax(kmax, dpp<0x128>(kmax, kmax));
+                    // the ballots cover the whole wavefront: keep this
+                    // group's lanes (its DPP row) only
+                    const unsigned long long rowMask = 0xffffull << (16 * p);
+                    int nm = 0, aStar = 0;
+                    unsigned long long mStar = 0ull;
+#pragma unroll
+                    for (int a = 0; a < NA; ++a) {
+                        const unsigned long long mk = __ballot(keyv[a] == kmax) & rowMask;
+                        nm += __builtin_popcountll(mk);
+                        if (mk) {
+                            aStar = a;
+                            mStar = mk;
+                        }
+                    }
+                    int Pw, prow;
+                    double dj;
+                    if (nm == 1) {
+                        const int lStar = __builtin_ctzll(mStar);
+                        prow = (lStar & 15) + kPR * aStar;
+                        dj = readlane(row_pick<NA, NC, bK>(A, aStar), lStar);
+                        Pw = -1;
+                    } else {
+                        double bkey = -2.0, bval = 0.0;
+                        int bpos = -1, brow = 0;
+#pragma unroll
+                        for (int a = 0; a < NA; ++a) {
+                            const int i = q.tr + kPR * a;
+                            const int ps = (keyv[a] > -2.0) ? L.pos[i] : -1;
+                            const bool better = keyv[a] > bkey || (keyv[a] == bkey && ps > bpos);
+                            bkey = better ? keyv[a] : bkey;
+                            bpos = better ? ps : bpos;
+                            brow = better ? i : brow;
+                            bval = better ? A[a][bK] : bval;
+                        }
+#define GFC_ROR_STEP(CTRL)                                                   \
+    {                                                                        \
+        const double ok = dpp<CTRL>(bkey, bkey);                             \
+        const int op = dppi<CTRL>(bpos, bpos);                               \
+        const int orw = dppi<CTRL>(brow, brow);                              \
+        const double ov = dpp<CTRL>(bval, bval);                             \
+        const bool better = ok > bkey || (ok == bkey && op > bpos);          \
+        bkey = better ? ok : bkey;                                           \
+        bpos = better ? op : bpos;                                           \
+        brow = better ? orw : brow;                                          \
+        bval = better ? ov : bval;                                           \
+    }
+                        GFC_ROR_STEP(0x121)
+                        GFC_ROR_STEP(0x122)
+                        GFC_ROR_STEP(0x124)
+                        GFC_ROR_STEP(0x128)
+#undef GFC_ROR_STEP
+                        // every lane of the row holds the result
+                        Pw = bpos;
+                        prow = brow;
+                        dj = bval;
+                    }
+                    const int aP = prow >> 4;
+                    if (dj == 0.0) {
+                        dj = kSmall;
+                        const bool me = q.tr == (prow & 15);
+#pragma unroll
+                        for (int a = 0; a < NA; ++a)
+                            A[a][bK] = (me && a == aP) ? kSmall : A[a][bK];
+                    }
+                    const double rDiag = 1.0 / dj;
+#pragma unroll
+                    for (int a = 0; a < NA; ++a) {
+                        const int i = q.tr + kPR * a;
+                        double l = 0.0;
+                        if (i < n && i != prow && !((pivoted >> a) & 1u)) {
+                            l = A[a][bK] * rDiag;
+                            A[a][bK] = l;
+                        }
+                        Lb[p * NV + i] = l;
+                    }
+                    if (q.tr == 0) {
+                        if (Pw < 0) Pw = L.pos[prow];
+                        const int qrow = L.perm[k];
+                        L.perm[Pw] = qrow;
+                        L.pos[qrow] = Pw;
+                        L.perm[k] = prow;
+                        L.pos[prow] = k;
+                        L.ppiv[par][p] = prow;
+                        L.rdiag[k] = rDiag;
+                    }
+                }
+                wave_fence();
+                const int prow = __builtin_amdgcn_readfirstlane(L.ppiv[par][p]);
+                const int aP = prow >> 4;
+                const bool own = q.tr == (prow & 15);
+                // the pivot row's entries of the later panel columns
+                if (own && q.g > p) Ub[q.g] = row_pick<NA, NC, bK>(A, aP);
+                wave_fence();
+                if (q.g > p) {
+                    const double u = Ub[q.g];
+#pragma unroll
+                    for (int a = 0; a < NA; ++a)
+                        A[a][bK] = fma(-Lb[p * NV + q.tr + kPR * a], u, A[a][bK]);
+                }
+                if (own) pivoted |= 1u << aP;
+                wave_fence();
+            }
+        }
+        __syncthreads();
+        int prw[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            prw[p] = p < np ? __builtin_amdgcn_readfirstlane(L.ppiv[par][p]) : -1;
+            if (p < np && q.w != wP && q.tr == (prw[p] & 15)) pivoted |= 1u << (prw[p] >> 4);
+        }
+        // trailing columns of this thread: register columns after bK, and
+        // bK itself in the thread columns after the panel's
+        const bool liveK = q.tc > 4 * wP + 3;
+        // U rows: U[p][j] = A[piv_p][j] - sum_{s<p} L[piv_p][s] U[s][j], by
+        // the lane holding row piv_p, pivot after pivot
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            if (p < np) {
+                const int prow = prw[p];
+                if (q.tr == (prow & 15)) {
+                    double ls[4];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) ls[s] = (s < p) ? Lb[s * NV + prow] : 0.0;
+                    double ur[NC];
+                    row_vals<NA, NC>(A, prow >> 4, ur);
+#pragma unroll
+                    for (int b = bK; b < NC; ++b) {
+                        if (b == bK && !liveK) continue;
+                        double u = ur[b];
+#pragma unroll
+                        for (int s = 0; s < 4; ++s)
+                            if (s < p) u = fma(-ls[s], Ub[(s * 4 + q.g) * NC + b], u);
+                        Ub[(p * 4 + q.g) * NC + b] = u;
+                    }
+                }
+                wave_fence();
+            }
+        }
+        // rank-4 update of every row
+        if constexpr (MFMA) {
+            // full 4-column tiles by MFMA: lane (g, tr) supplies
+            // B[p = g][n = tr] = L[row][g] and A[m = tr][p = g] =
+            // -U[g][column (tr / 4, tr % 4) of the tile]
+            constexpr int kFull = NC / 4 * 4;
+#pragma unroll
+            for (int b0 = (bK / 4) * 4; b0 < kFull; b0 += 4) {
+                const int bu = b0 + (q.tr >> 2);   // register column of this lane's U operand
+                const int gu = q.tr & 3;           // its thread-column group
+                const bool uLive = q.g < np && (bu > bK || (bu == bK && 4 * q.w + gu > 4 * wP + 3));
+                const double aop = uLive ? -Ub[(q.g * 4 + gu) * NC + bu] : 0.0;
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                    const double bop = (q.g < np) ? Lb[q.g * NV + q.tr + kPR * a] : 0.0;
+                    dbl4 c;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) c[r] = A[a][b0 + r];
+                    const dbl4 d = __builtin_amdgcn_mfma_f64_16x16x4f64(aop, bop, c, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int b = b0 + r;
+                        if (b > bK) A[a][b] = d[r];
+                        else if (b == bK) A[a][b] = liveK ? d[r] : A[a][b];
+                    }
+                }
+            }
+            // the columns beyond the last full tile with vector FMAs
+            rank4_valu<NA, NC, bK, (bK > kFull ? bK : kFull)>(A, Lb, Ub, np, liveK, q);
+        } else {
+            rank4_valu<NA, NC, bK, bK>(A, Lb, Ub, np, liveK, q);
+        }
+        par ^= 1;
+    }
+}
+
+template <int NA, int NC, int LUV, int B = 0>
 __device__ __forceinline__ void lu_all(double (&A)[NA][NC], int n, BlockLDS<NA, NC>& L,
-                                       const Coord& q, unsigned& pivoted) {
+                                       const Coord& q, unsigned& pivoted, int& par) {
     if constexpr (B < NC) {
-        if (kPC * B < n) lu_cols<NA, NC, B>(A, n, L, q, pivoted);
-        lu_all<NA, NC, B + 1>(A, n, L, q, pivoted);
+        if (kPC * B < n) {
+            if constexpr (LUV == 0) lu_cols<NA, NC, B>(A, n, L, q, pivoted);
+            else lu_panels<NA, NC, B, LUV == 2>(A, n, L, q, pivoted, par);
+        }
+        lu_all<NA, NC, LUV, B + 1>(A, n, L, q, pivoted, par);
     }
 }
 
@@ -651,7 +923,7 @@ __device__ __forceinline__ void lu_all(double (&A)[NA][NC], int n, BlockLDS<NA, 
 // return L.pos / L.perm map rows to pivot positions, L.rdiag holds the
 // reciprocal pivots and the diagonal 32 x 32 blocks of the factors
 // (position order) are in L.R.
-template <int NA, int NC>
+template <int NA, int NC, int LUV>
 __device__ __forceinline__ void blk_lu(double (&A)[NA][NC], int n, BlockLDS<NA, NC>& L,
                                        double* LUg, const Coord& q0) {
     const Coord q = opaque(q0);
@@ -679,7 +951,8 @@ __device__ __forceinline__ void blk_lu(double (&A)[NA][NC], int n, BlockLDS<NA, 
     for (int i = q.tid; i < n; i += kBT) L.vv[i] = 1.0 / __longlong_as_double((long long)L.vmax[i]);
     __syncthreads();
     unsigned pivoted = 0u;
-    lu_all(A, n, L, q, pivoted);
+    int par = 0;
+    lu_all<NA, NC, LUV>(A, n, L, q, pivoted, par);
     __syncthreads();
     // diagonal blocks of the factors in position order: R[(b*32 + r)*33 + c]
 #pragma unroll
@@ -863,7 +1136,7 @@ __device__ __forceinline__ void blk_solve(const double* LUg, int n, BlockLDS<NA,
 
 // ---------------------------------------------------------------- kernel --
 
-template <int NA, int NC>
+template <int NA, int NC, int LUV>
 __global__ void __launch_bounds__(kBT)
 block_solve_kernel(SolveArgs A, int* queue, double* scratch, long perWG) {
     __shared__ BlockLDS<NA, NC> L;
@@ -941,7 +1214,7 @@ block_solve_kernel(SolveArgs A, int* queue, double* scratch, long perWG) {
                                     Am[a][b] = v;
                                 }
                             }
-                            blk_lu(Am, n, L, LUg, q);
+                            blk_lu<NA, NC, LUV>(Am, n, L, LUg, q);
                         }
                         // Rosenbrock stages (gpuRosenbrock{12,23,34}.H::solve)
 #pragma unroll 1
@@ -1047,7 +1320,13 @@ block_solve_kernel(SolveArgs A, int* queue, double* scratch, long perWG) {
 
 static_assert(BlockLDS<8, 4>::RSZ == block_lds_doubles(128), "LDS region of n <= 128");
 static_assert(BlockLDS<13, 7>::RSZ == block_lds_doubles(208), "LDS region of n <= 208");
-template __global__ void block_solve_kernel<8, 4>(SolveArgs, int*, double*, long);
-template __global__ void block_solve_kernel<13, 7>(SolveArgs, int*, double*, long);
+// factorisation variants: 0 column by column, 1 four-column panels with
+// vector FMAs, 2 four-column panels with MFMA trailing updates
+template __global__ void block_solve_kernel<8, 4, 0>(SolveArgs, int*, double*, long);
+template __global__ void block_solve_kernel<8, 4, 1>(SolveArgs, int*, double*, long);
+template __global__ void block_solve_kernel<8, 4, 2>(SolveArgs, int*, double*, long);
+template __global__ void block_solve_kernel<13, 7, 0>(SolveArgs, int*, double*, long);
+template __global__ void block_solve_kernel<13, 7, 1>(SolveArgs, int*, double*, long);
+template __global__ void block_solve_kernel<13, 7, 2>(SolveArgs, int*, double*, long);
 
 } // namespace gfc

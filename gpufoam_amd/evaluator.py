@@ -94,6 +94,17 @@ class Evaluator:
             self._h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
 
+    def set_factorization(self, variant: int) -> None:
+        """LU variant of the workgroup-per-cell kernel: _capi.LU_COLUMN,
+        LU_PANEL (bit-identical factors, one barrier per four columns) or
+        LU_PANEL_MFMA (rank-4 trailing updates on the matrix cores)."""
+        _capi.check(_capi.lib().gfc_evaluator_set_factorization(self._h, variant))
+
+    def factorization(self) -> int:
+        v = C.c_int()
+        _capi.check(_capi.lib().gfc_evaluator_get_factorization(self._h, C.byref(v)))
+        return v.value
+
     # -------------------------------------------------------------- solve
     def solve(self, deltaT: float, Y0, dtChem, Ynew=None, RR=None, rho=None,
               deltaTMin=None, stats=None, deltaTChemMax: float = 1e300,

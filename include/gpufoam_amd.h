@@ -288,6 +288,21 @@ int gfc_evaluator_set_batched_shape(gfc_evaluator* ev, int cellsPerWave,
 int gfc_evaluator_get_batched_shape(const gfc_evaluator* ev, int* cellsPerWave,
                                     int* wavesPerBlock, int* chunk);
 
+/* Factorisation of the workgroup-per-cell kernel (64 < nSpecie+1 <= 208),
+ * which factorises the Rosenbrock matrix with the row-scaled partial
+ * pivoting of gpuFoam's LUDecompose (ludecompose.H:14-95):
+ * GFC_LU_COLUMN eliminates column by column, one workgroup barrier per
+ * column; GFC_LU_PANEL factorises four columns at a time in one wavefront
+ * (one barrier per four columns) and applies the rank-4 trailing update with
+ * vector FMAs (same arithmetic, factors bit-identical to GFC_LU_COLUMN);
+ * GFC_LU_PANEL_MFMA applies that update with v_mfma_f64_16x16x4f64 (other
+ * rounding of the four-term sums).  Other kernels ignore the setting. */
+#define GFC_LU_COLUMN 0
+#define GFC_LU_PANEL 1
+#define GFC_LU_PANEL_MFMA 2
+int gfc_evaluator_set_factorization(gfc_evaluator* ev, int variant);
+int gfc_evaluator_get_factorization(const gfc_evaluator* ev, int* variant);
+
 /* ---- chemistry queries at the current state ----------------------------
  * The non-ODE queries of gpuFoam's chemistry model, which the reference
  * evaluates on the CPU with OpenFOAM's Reaction objects

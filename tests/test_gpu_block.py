@@ -202,3 +202,45 @@ def test_block_kernel_reports_max_steps():
     _, _, _, st, _ = ev.solve_host(1e-6, Y0, dt0, batched=True)
     assert np.array_equal(st[0], ref[3][0])
     assert (st[0] & 1).any()
+
+
+@pytest.mark.parametrize("which", ["syn200", "syn100"])
+def test_block_panel_factorization_is_bit_identical(which):
+    """The four-column panel LU does every entry's arithmetic in the
+    column-by-column order, so the whole solve (states, chemical time steps,
+    step counts, RR) is bit-identical between the two."""
+    from gpufoam_amd import _capi
+    if which == "syn200":
+        m, Y0, dt0, c = make_states("syn200", 32)
+    else:
+        m = _syn100()
+        Y0, dt0 = syn_states(m, 12, seed=11)
+        c = dict(solver="Rosenbrock34", relTol=1e-6, absTol=1e-10, deltaT=1e-6)
+    ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
+    rho = np.linspace(0.4, 1.2, Y0.shape[1])
+    runs = []
+    for lu in (_capi.LU_COLUMN, _capi.LU_PANEL):
+        print(f"case {which} n={Y0.shape[1]} factorization {lu}", flush=True)
+        runs.append(_device_run(m, Y0, dt0, c, ode, rho, batched=True, factorization=lu))
+    for a, b in zip(*runs):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+
+
+@pytest.mark.parametrize("which", ["syn200", "syn100"])
+def test_block_mfma_factorization_matches_oracle(which):
+    """Rank-4 trailing updates on the matrix cores (v_mfma_f64_16x16x4f64)
+    round the four-term sums differently: the solve is held to the oracle
+    with the bound of the module docstring."""
+    from gpufoam_amd import _capi
+    if which == "syn200":
+        m, Y0, dt0, c = make_states("syn200", 32)
+    else:
+        m = _syn100()
+        Y0, dt0 = syn_states(m, 12, seed=11)
+        c = dict(solver="Rosenbrock34", relTol=1e-6, absTol=1e-10, deltaT=1e-6)
+    rho = np.linspace(0.4, 1.2, Y0.shape[1])
+    ode, ref = _oracle(m, Y0, dt0, c, rho)
+    print(f"case {which} n={Y0.shape[1]} factorization mfma", flush=True)
+    got = _device_run(m, Y0, dt0, c, ode, rho, batched=True,
+                      factorization=_capi.LU_PANEL_MFMA)
+    _check_conditioned(m, got, ref, Y0, dt0, c, ode, rho=rho)

@@ -48,12 +48,14 @@ def _oracle_run(cfg, n, **kw):
 
 
 def _device_run(m, Y0, dt0, c, ode, rho, batched, graph=False, deltaTChemMax=1e300,
-                Treact=0.0, cells_per_wave=None, lane=False):
+                Treact=0.0, cells_per_wave=None, lane=False, factorization=None):
     from gpufoam_amd.evaluator import Evaluator
     n = Y0.shape[1]
     ev = Evaluator(m, ode, Treact, max_cells=n)
     if cells_per_wave is not None:
         ev.set_batched_shape(cells_per_wave)
+    if factorization is not None:
+        ev.set_factorization(factorization)
     dY0 = torch.tensor(Y0, device="cuda")
     dYn = torch.empty_like(dY0)
     dRR = torch.zeros((m.n_species, n), dtype=torch.float64, device="cuda")

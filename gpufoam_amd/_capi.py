@@ -39,6 +39,8 @@ GRAPH = 0x4
 LANE_PER_CELL = 0x8
 # factorisation of the workgroup-per-cell kernel
 LU_COLUMN, LU_PANEL, LU_PANEL_MFMA = 0, 1, 2
+# kernel families of the device-level evaluations
+SHAPE_AUTO, SHAPE_WAVE, SHAPE_TEAM16, SHAPE_WORKGROUP = 0, 1, 4, 512
 
 
 class GfcError(RuntimeError):
@@ -115,6 +117,9 @@ def lib() -> C.CDLL:
             "gfc_evaluator_get_batched_shape": (C.c_int, [vp, _ip, _ip, _ip]),
             "gfc_evaluator_set_factorization": (C.c_int, [vp, C.c_int]),
             "gfc_evaluator_get_factorization": (C.c_int, [vp, _ip]),
+            "gfc_eval_derivatives": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp]),
+            "gfc_eval_jacobian": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]),
+            "gfc_eval_lu_solve": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]),
             "gfc_evaluator_param_count": (C.c_int, [vp]),
             "gfc_evaluator_get_rate_parameters": (C.c_int, [vp, _dp]),
             "gfc_evaluator_set_rate_parameters": (C.c_int, [vp, _dp]),
@@ -154,7 +159,8 @@ def exported_symbols() -> list[str]:
         "gfc_solver_from_name", "gfc_ode_inputs_default", "gfc_evaluator_create",
         "gfc_evaluator_destroy", "gfc_solve", "gfc_evaluator_set_batched_shape",
         "gfc_evaluator_get_batched_shape", "gfc_evaluator_set_factorization",
-        "gfc_evaluator_get_factorization", "gfc_evaluator_param_count",
+        "gfc_evaluator_get_factorization", "gfc_eval_derivatives", "gfc_eval_jacobian",
+        "gfc_eval_lu_solve", "gfc_evaluator_param_count",
         "gfc_evaluator_get_rate_parameters", "gfc_evaluator_set_rate_parameters",
         "gfc_adjoint_set_checkpoint_interval", "gfc_adjoint_checkpoint_interval",
         "gfc_adjoint_forward", "gfc_adjoint_backward", "gfc_adjoint_replay",

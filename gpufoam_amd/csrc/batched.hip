@@ -28,6 +28,12 @@ __global__ void team16_solve_kernel(SolveArgs A, int* queue);
 template <int NA, int NC, int LUV>
 __global__ void block_solve_kernel(SolveArgs A, int* queue, double* scratch, long perWG);
 __global__ void lane_solve_kernel(SolveArgs A, double* scratch, long per);
+template <int N>
+__global__ void team_eval_kernel(EvalArgs E);
+template <int N>
+__global__ void team16_eval_kernel(EvalArgs E);
+template <int NA, int NC>
+__global__ void block_eval_kernel(EvalArgs E, double* scratch, long perWG);
 
 namespace {
 
@@ -216,7 +222,8 @@ static void choose_plan(gfc_evaluator* ev, int nCells, Plan& pl, int& cellsPerBl
         pl.grid = b->numCUs * perCU;
         pl.key = 2000 + N16;
         cellsPerBlock = kLanesPerWave / 16;
-    } else if (team && ev->dmTeam.teamWidth <= kTeamWidth) {
+    } else if (team && ev->dmTeam.teamWidth <= kTeamWidth &&
+               ev->dmTeam.nSets <= kTeamMaxSets) {
         int perCU = resident_blocks(team);
         if (ev->wavesPerBlock > 0 && ev->wavesPerBlock < perCU) perCU = ev->wavesPerBlock;
         pl.team = team;
@@ -347,9 +354,114 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
     return GFC_OK;
 }
 
+// Device-level evaluation (derivative, Jacobian, LU solve) with the
+// functions of one kernel family: shape 0 = the family the batched solve
+// uses for this evaluator, 1 = wavefront per cell, 4 = 16-lane teams,
+// 512 = workgroup per cell.
+int device_eval(gfc_evaluator* ev, int shape, EvalArgs E, hipStream_t s) {
+    auto* b = static_cast<BatchedState*>(ev->batched);
+    const int n = ev->nS + 1;
+    if (ev->dmTeam.extended) {
+        setLastError("device evaluation: the batched kernels evaluate gpuFoam's four rate "
+                     "types only (this mechanism runs one lane per cell)");
+        return GFC_ERR_UNSUPPORTED;
+    }
+    if (shape == 0) {
+        Plan pl;
+        int perBlock = 1;
+        choose_plan(ev, ev->maxCells, pl, perBlock);
+        shape = pl.blk ? 512 : perBlock > 1 ? 4 : pl.team ? 1 : 64;
+    }
+    E.m = ev->dmTeam;
+    if (E.nCells == 0) return GFC_OK;
+    if (shape == 1) {
+        int N = 0;
+        if (!team_kernel_for(n, N) || ev->dmTeam.teamWidth > kTeamWidth ||
+            ev->dmTeam.nSets > kTeamMaxSets) {
+            setLastError("device evaluation: no wavefront kernel for this mechanism "
+                         "(nSpecie+1 <= 64, at most 3 species per reaction side, at most "
+                         "128 distinct third-body efficiency sets)");
+            return GFC_ERR_UNSUPPORTED;
+        }
+        void (*k)(EvalArgs) = N == 11 ? team_eval_kernel<11> : N == 32 ? team_eval_kernel<32>
+                            : N == 54 ? team_eval_kernel<54> : team_eval_kernel<64>;
+        hipLaunchKernelGGL(k, dim3(E.nCells), dim3(64), 0, s, E);
+    } else if (shape == 4) {
+        int N = 0;
+        if (!team16_kernel_for(ev->dmTeam, N)) {
+            setLastError("device evaluation: no 16-lane kernel for this mechanism "
+                         "(nSpecie+1 <= 16, at most 16 efficiency sets, 3 species per side)");
+            return GFC_ERR_UNSUPPORTED;
+        }
+        void (*k)(EvalArgs) = N == 11 ? team16_eval_kernel<11> : team16_eval_kernel<16>;
+        hipLaunchKernelGGL(k, dim3((E.nCells + 3) / 4), dim3(64), 0, s, E);
+    } else if (shape == 512) {
+        if (b->blkNA == 0) {
+            setLastError("device evaluation: no workgroup kernel for this mechanism "
+                         "(64 < nSpecie+1 <= 208)");
+            return GFC_ERR_UNSUPPORTED;
+        }
+        void (*k)(EvalArgs, double*, long) =
+            b->blkNA == 8 ? block_eval_kernel<8, 4> : block_eval_kernel<13, 7>;
+        const int grid = b->blkGrid < E.nCells ? b->blkGrid : E.nCells;
+        hipLaunchKernelGGL(k, dim3(grid), dim3(kBlockThreads), 0, s, E, b->blkScratch,
+                           b->blkPerWG);
+    } else {
+        setLastError("device evaluation: shape must be 0, 1, 4 or 512");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    return hipFail(hipGetLastError(), "device evaluation launch") ? GFC_ERR_HIP : GFC_OK;
+}
+
 } // namespace gfc
 
 extern "C" {
+int gfc_eval_derivatives(gfc_evaluator* ev, int shape, int nCells, int ldY, const double* Y,
+                         double* dYdt, void* stream) {
+    if (!ev || nCells < 0 || ldY < nCells || (nCells > 0 && (!Y || !dYdt))) {
+        gfc::setLastError("gfc_eval_derivatives: bad argument");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    gfc::EvalArgs E{};
+    E.mode = 0;
+    E.nCells = nCells;
+    E.ldY = ldY;
+    E.Y = Y;
+    E.dYdt = dYdt;
+    return gfc::device_eval(ev, shape, E, static_cast<hipStream_t>(stream));
+}
+int gfc_eval_jacobian(gfc_evaluator* ev, int shape, int nCells, int ldY, const double* Y,
+                      double* dYdt, double* J, void* stream) {
+    if (!ev || nCells < 0 || ldY < nCells || (nCells > 0 && (!Y || !dYdt || !J))) {
+        gfc::setLastError("gfc_eval_jacobian: bad argument");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    gfc::EvalArgs E{};
+    E.mode = 1;
+    E.nCells = nCells;
+    E.ldY = ldY;
+    E.Y = Y;
+    E.dYdt = dYdt;
+    E.J = J;
+    return gfc::device_eval(ev, shape, E, static_cast<hipStream_t>(stream));
+}
+int gfc_eval_lu_solve(gfc_evaluator* ev, int shape, int nCells, int ldB, const double* M,
+                      const double* b, double* x, void* stream) {
+    if (!ev || nCells < 0 || ldB < nCells || (nCells > 0 && (!M || !b || !x))) {
+        gfc::setLastError("gfc_eval_lu_solve: bad argument");
+        return GFC_ERR_INVALID_ARGUMENT;
+    }
+    gfc::EvalArgs E{};
+    E.mode = 2;
+    E.nCells = nCells;
+    E.ldY = ldB;
+    E.M = M;
+    E.b = b;
+    E.x = x;
+    E.lu = ev->factorization;
+    return gfc::device_eval(ev, shape, E, static_cast<hipStream_t>(stream));
+}
+
 int gfc_evaluator_set_batched_shape(gfc_evaluator* ev, int cellsPerWave,
                                     int wavesPerBlock, int chunk) {
     if (!ev || (cellsPerWave != 0 && cellsPerWave != 1 && cellsPerWave != 4 &&

@@ -1318,6 +1318,74 @@ block_solve_kernel(SolveArgs A, int* queue, double* scratch, long perWG) {
     }
 }
 
+// Device-level evaluations with the workgroup kernel's own physics and LU;
+// block b takes cells b, b + gridDim.x, ... and its scratch slot as in the
+// solve.
+template <int NA, int NC>
+__global__ void __launch_bounds__(kBT)
+block_eval_kernel(EvalArgs E, double* scratch, long perWG) {
+    __shared__ BlockLDS<NA, NC> L;
+    const Coord q = coord();
+    const int t = q.tid;
+    const DeviceMech& m = E.m;
+    const int nS = m.nS, n = nS + 1;
+    const long ld = E.ldY;
+    const long nRp = ((long)m.nR + 63) / 64 * 64;
+    double* Jg = scratch + (long)blockIdx.x * perWG;
+    double* LUg = Jg + (long)NA * NC * kBT;
+    int tog = 0;
+    for (int cell = blockIdx.x; cell < E.nCells; cell += gridDim.x) {
+        if (E.mode == 2) {
+            const double* M = E.M + (long)cell * n * n;
+            double Am[NA][NC];
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                const int i = q.tr + kPR * a;
+#pragma unroll
+                for (int b = 0; b < NC; ++b) {
+                    const int j = q.tc + kPC * b;
+                    Am[a][b] = (i < n && j < n) ? M[(long)i * n + j] : 0.0;
+                }
+            }
+            if (E.lu == GFC_LU_PANEL) blk_lu<NA, NC, 1>(Am, n, L, LUg, q);
+            else if (E.lu == GFC_LU_PANEL_MFMA) blk_lu<NA, NC, 2>(Am, n, L, LUg, q);
+            else blk_lu<NA, NC, 0>(Am, n, L, LUg, q);
+            if (t < n) L.rhs[t] = E.b[t * ld + cell];
+            __syncthreads();
+            blk_solve(LUg, n, L, L.rhs, L.k1, q);
+            if (t < n) E.x[t * ld + cell] = L.k1[t];
+            __syncthreads();
+            continue;
+        }
+        if (t < n) L.y[t] = E.Y[t * ld + cell];
+        const double p = E.Y[(long)(nS + 1) * ld + cell];
+        __syncthreads();
+        if (E.mode == 0) {
+            blk_derivatives(m, p, L.y, L.dydx, L, tog, q);
+            if (t < n) E.dYdt[t * ld + cell] = L.dydx[t];
+        } else {
+            blk_jacobian(m, p, L, nRp, Jg, tog, q);
+            __syncthreads();
+            if (t < n) E.dYdt[t * ld + cell] = L.dfdx[t];
+            double* Jc = E.J + (long)cell * n * n;
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                const int i = q.tr + kPR * a;
+#pragma unroll
+                for (int b = 0; b < NC; ++b) {
+                    const int j = q.tc + kPC * b;
+                    if (i < n && j < n)
+                        Jc[(long)i * n + j] = wave_slots<NA * NC>(Jg, q)[(a * NC + b) * 64 + q.lane];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template __global__ void block_eval_kernel<8, 4>(EvalArgs, double*, long);
+template __global__ void block_eval_kernel<13, 7>(EvalArgs, double*, long);
+
 static_assert(BlockLDS<8, 4>::RSZ == block_lds_doubles(128), "LDS region of n <= 128");
 static_assert(BlockLDS<13, 7>::RSZ == block_lds_doubles(208), "LDS region of n <= 208");
 // factorisation variants: 0 column by column, 1 four-column panels with

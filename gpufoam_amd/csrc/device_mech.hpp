@@ -115,6 +115,9 @@ constexpr int kSetsT = 16;
 
 // Record layout of the wavefront kernel (at most kTeamWidth species per side).
 constexpr int kTeamWidth = 3;
+// Third-body concentrations the wavefront kernel holds in LDS (one per
+// distinct efficiency set); mechanisms with more sets take another kernel.
+constexpr int kTeamMaxSets = 128;
 enum RecD {
     rdLnA, rdBeta, rdTa, rdSgnA,       // k (Arrhenius / third body) or k0
     rdLnAi, rdBetai, rdTai, rdSgnAi,   // kInf of fall-off rates

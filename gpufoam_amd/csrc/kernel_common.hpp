@@ -86,6 +86,24 @@ struct SolveArgs {
     int* stats;                    // [3*ldY] or null
 };
 
+// Device-level evaluation of one kernel family's physics (the functions its
+// solve uses), one cell per wavefront / 16-lane team / workgroup:
+// mode 0 the derivative, 1 the Jacobian with the derivative computed
+// alongside it, 2 the factorisation of a given matrix and one solve.
+struct EvalArgs {
+    DeviceMech m;
+    int mode;
+    int nCells;
+    long ldY;
+    const double* Y;  // [nS+2][ldY]: species, T, p (modes 0, 1)
+    double* dYdt;     // [n][ldY] (modes 0, 1)
+    double* J;        // [nCells][n][n] row-major (mode 1)
+    const double* M;  // [nCells][n][n] row-major (mode 2)
+    const double* b;  // [n][ldY] (mode 2)
+    double* x;        // [n][ldY] (mode 2)
+    int lu;           // workgroup kernel's factorisation variant (mode 2)
+};
+
 // Capacity of the workgroup-per-cell kernel (block_kernel.hip): n =
 // nSpecie+1 in (64, kBlockMaxN], at most kBlockMaxR reactions (net rates in
 // LDS) and kBlockMaxSets distinct third-body efficiency sets.

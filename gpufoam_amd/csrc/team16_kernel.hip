@@ -230,6 +230,53 @@ team16_record_kernel(SolveArgs A, int* queue, T16Record R) {
     t16_run<N, true>(A, queue, R, Ls);
 }
 
+// Device-level evaluations with the 16-lane teams' own physics and LU: team
+// t of block b takes cell 4b + t (grid = ceil(nCells / 4)).
+template <int N>
+__global__ void __launch_bounds__(64) team16_eval_kernel(EvalArgs E) {
+    __shared__ Team16LDS<N> Ls[kLanes / kTW];
+    const int lane = threadIdx.x;
+    const int team = lane >> 4, mem = lane & 15;
+    const int cell = (int)blockIdx.x * (kLanes / kTW) + team;
+    if (cell >= E.nCells) return; // the teams are independent
+    Team16LDS<N>& L = Ls[team];
+    const DeviceMech& m = E.m;
+    const int nS = m.nS, n = nS + 1;
+    const long ld = E.ldY;
+    if (E.mode == 2) {
+        const double* M = E.M + (long)cell * n * n;
+        double row[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            double v = (k == mem) ? 1.0 : 0.0;  // identity rows beyond n
+            if (mem < n) v = (k < n) ? M[mem * n + k] : 0.0;
+            row[k] = v;
+        }
+        int ps, plv;
+        double rd;
+        t16_lu<N>(row, n, lane, mem, ps, rd, plv);
+        const double bl = (mem < n) ? E.b[mem * ld + cell] : 0.0;
+        const double xl = t16_solve<N>(row, lane, mem, ps, rd, plv, bl);
+        if (mem < n) E.x[mem * ld + cell] = xl;
+        return;
+    }
+    const double y = (mem < n) ? E.Y[mem * ld + cell] : 0.0;
+    const double p = E.Y[(long)(nS + 1) * ld + cell];
+    const double T = team_bcast(y, lane, nS);
+    if (E.mode == 0) {
+        const double d = t16_derivatives<N>(m, p, T, y, L, lane, mem);
+        if (mem < n) E.dYdt[mem * ld + cell] = d;
+    } else {
+        const double d = t16_jacobian<N>(m, p, T, y, L, lane, mem);
+        if (mem < n) E.dYdt[mem * ld + cell] = d;
+        wave_sync();
+        double* Jc = E.J + (long)cell * n * n;
+        for (int e = mem; e < n * n; e += kTW) Jc[e] = L.J[(e / n) * Team16LDS<N>::P + e % n];
+    }
+}
+
+template __global__ void team16_eval_kernel<11>(EvalArgs);
+template __global__ void team16_eval_kernel<16>(EvalArgs);
 template __global__ void team16_solve_kernel<11>(SolveArgs, int*);
 template __global__ void team16_solve_kernel<16>(SolveArgs, int*);
 template __global__ void team16_record_kernel<11>(SolveArgs, int*, T16Record);

@@ -43,7 +43,7 @@ struct TeamLDS {
     double Ha[kLanes];
     double colA[kLanes];            // rhoM / W_j
     double colB[kLanes];            // rhoM * v_j
-    double M[kLanes];               // third-body concentration per set
+    double M[kTeamMaxSets];         // third-body concentration per set
     double gL[kLanes], gH[kLanes];  // W_i Gstd_i with the low / high polynomial
     double hL[kLanes], hH[kLanes];  // W_i Ha_i with the low / high polynomial
     double Wd[kMaxSets * kLanes];   // dense Jacobian weights per set, specie
@@ -616,6 +616,52 @@ team_solve_kernel(SolveArgs A, int* queue) {
     }
 }
 
+// Device-level evaluations with this kernel's own physics and LU (one cell
+// per wavefront; grid = nCells).
+template <int N>
+__global__ void __launch_bounds__(64) team_eval_kernel(EvalArgs E) {
+    __shared__ TeamLDS<N> L;
+    const int lane = threadIdx.x;
+    const int cell = blockIdx.x;
+    if (cell >= E.nCells) return;
+    const DeviceMech& m = E.m;
+    const int nS = m.nS, n = nS + 1;
+    const long ld = E.ldY;
+    if (E.mode == 2) {
+        const double* M = E.M + (long)cell * n * n;
+        double row[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            double v = (k == lane) ? 1.0 : 0.0;  // identity rows beyond n
+            if (lane < n) v = (k < n) ? M[lane * n + k] : 0.0;
+            row[k] = v;
+        }
+        int ps, plv;
+        double rd;
+        team_lu<N>(row, n, lane, ps, rd, plv);
+        const double bl = (lane < n) ? E.b[lane * ld + cell] : 0.0;
+        const double xl = team_solve<N>(row, lane, ps, rd, plv, bl);
+        if (lane < n) E.x[lane * ld + cell] = xl;
+        return;
+    }
+    const double y = (lane < n) ? E.Y[lane * ld + cell] : 0.0;
+    const double p = E.Y[(long)(nS + 1) * ld + cell];
+    const double T = readlane(y, nS);
+    if (E.mode == 0) {
+        const double d = team_derivatives<N>(m, p, T, y, L, lane);
+        if (lane < n) E.dYdt[lane * ld + cell] = d;
+    } else {
+        const double d = team_jacobian<N>(m, p, T, y, L, lane);
+        if (lane < n) E.dYdt[lane * ld + cell] = d;
+        double* Jc = E.J + (long)cell * n * n;
+        for (int e = lane; e < n * n; e += kLanes) Jc[e] = L.J[(e / n) * TeamLDS<N>::P + e % n];
+    }
+}
+
+template __global__ void team_eval_kernel<11>(EvalArgs);
+template __global__ void team_eval_kernel<32>(EvalArgs);
+template __global__ void team_eval_kernel<54>(EvalArgs);
+template __global__ void team_eval_kernel<64>(EvalArgs);
 template __global__ void team_solve_kernel<11>(SolveArgs, int*);
 template __global__ void team_solve_kernel<32>(SolveArgs, int*);
 template __global__ void team_solve_kernel<54>(SolveArgs, int*);

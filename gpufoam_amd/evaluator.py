@@ -227,6 +227,44 @@ class Evaluator:
         _capi.check(_capi.lib().gfc_adjoint_replay(
             self._h, _ptr(Y0), Y0.shape[1], _ptr(Ynew), stream))
 
+    # ------------------------------------------------- device evaluations
+    # One kernel family's own derivative, Jacobian and LU solve
+    # (gpuODESystem.H:164-281, ludecompose.H:9-140) on device tensors; shape
+    # as _capi.SHAPE_* (0: the family the batched solve uses).
+    def eval_derivatives(self, Y, shape: int = 0, stream=None):
+        """Y [nS+2, n] -> dYdt [nS+1, n] (species, T)."""
+        import torch
+        n = Y.shape[1]
+        _check_device("Y", Y, self.nE, n)
+        d = torch.empty((self.nS + 1, n), dtype=torch.float64, device=Y.device)
+        _capi.check(_capi.lib().gfc_eval_derivatives(
+            self._h, shape, n, n, _ptr(Y), _ptr(d), self._stream(stream)))
+        return d
+
+    def eval_jacobian(self, Y, shape: int = 0, stream=None):
+        """Y [nS+2, n] -> (dYdt [nS+1, n], J [n, nS+1, nS+1])."""
+        import torch
+        n = Y.shape[1]
+        _check_device("Y", Y, self.nE, n)
+        m = self.nS + 1
+        d = torch.empty((m, n), dtype=torch.float64, device=Y.device)
+        J = torch.empty((n, m, m), dtype=torch.float64, device=Y.device)
+        _capi.check(_capi.lib().gfc_eval_jacobian(
+            self._h, shape, n, n, _ptr(Y), _ptr(d), _ptr(J), self._stream(stream)))
+        return d, J
+
+    def eval_lu_solve(self, M, b, shape: int = 0, stream=None):
+        """M [n, nS+1, nS+1] (row-major per cell), b [nS+1, n] -> x [nS+1, n]."""
+        import torch
+        n = b.shape[1]
+        m = self.nS + 1
+        _check_device("b", b, m, n)
+        _check_device("M", M.reshape(n * m, m), n * m, m, device=b.device)
+        x = torch.empty_like(b)
+        _capi.check(_capi.lib().gfc_eval_lu_solve(
+            self._h, shape, n, n, _ptr(M), _ptr(b), _ptr(x), self._stream(stream)))
+        return x
+
     # ------------------------------------------------------------ queries
     # gpuChemistryModel's queries at the current state
     # (gpuChemistryModel.C:146-466) on device tensors: Y [nS+2, ld] SoA,

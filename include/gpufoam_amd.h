@@ -240,8 +240,9 @@ void gfc_evaluator_destroy(gfc_evaluator* ev);
  * GFC_HOST_POINTERS the call is asynchronous on `stream` and allocates
  * nothing.
  * Kernels: without GFC_BATCHED one launch covers the cells (the drop-in
- * form): one wavefront per cell (nSpecie+1 <= 64; four cells per wavefront
- * where the batched shape chooses that), one 512-thread workgroup per cell
+ * form): one wavefront per cell (nSpecie+1 <= 64, at most 3 species per
+ * reaction side and 128 distinct third-body efficiency sets; four cells per
+ * wavefront where the batched shape chooses that), one 512-thread workgroup per cell
  * (64 < nSpecie+1 <= 208) on a resident grid dealing cells round-robin, or
  * with GFC_LANE_PER_CELL one lane per cell.  GFC_BATCHED runs the same
  * per-cell kernels over a persistent work-stealing grid; both forms give
@@ -275,7 +276,8 @@ int gfc_solve_noalloc(gfc_evaluator* ev, double deltaT, double deltaTChemMax,
 /* Launch-shape control for the persistent batched form (the autotuner's
  * search space, gpufoam_amd/autotune.py).  cellsPerWave: 0 = automatic
  * (by mechanism size: four cells per wavefront for nSpecie+1 <= 16 from
- * 16384 cells on, one wavefront per cell up to 64, one 512-thread workgroup
+ * 16384 cells on, one wavefront per cell up to 64 (at most 128 efficiency
+ * sets), one 512-thread workgroup
  * per cell up to 208, else one lane per cell), 1 = one wavefront per cell,
  * 4 = four 16-lane cells per wavefront (nSpecie+1 <= 16, at most 16
  * third-body efficiency sets, at most 3 species per reaction side; other
@@ -302,6 +304,37 @@ int gfc_evaluator_get_batched_shape(const gfc_evaluator* ev, int* cellsPerWave,
 #define GFC_LU_PANEL_MFMA 2
 int gfc_evaluator_set_factorization(gfc_evaluator* ev, int variant);
 int gfc_evaluator_get_factorization(const gfc_evaluator* ev, int* variant);
+
+/* ---- device-level evaluations ---------------------------------------
+ * The physics and the LU of one batched kernel family run on their own,
+ * one cell per wavefront / 16-lane team / workgroup, as the solve runs them:
+ * the counterparts of the reference's device kernels that its unit tests
+ * call (gpuODESystem::derivatives and ::jacobian, gpuODESystem.H:164-281;
+ * LUDecompose + LUBacksubstitute, ludecompose.H:9-140; tested in
+ * unittest/tests/Test-gpuChemistry.C:521-691).  shape: 0 = the family the
+ * batched solve uses for this evaluator, GFC_SHAPE_WAVE (nSpecie+1 <= 64),
+ * GFC_SHAPE_TEAM16 (nSpecie+1 <= 16), GFC_SHAPE_WORKGROUP
+ * (64 < nSpecie+1 <= 208); a family that cannot take the mechanism, or a
+ * mechanism with rate types beyond gpuFoam's four, returns
+ * GFC_ERR_UNSUPPORTED.  Device pointers, asynchronous on `stream`.
+ * Y is the SoA state of gfc_solve ([nSpecie+2][ldY]); dYdt receives
+ * [n][ldY], n = nSpecie+1 (species, then T); J receives per cell the
+ * row-major n x n Jacobian d(dYdt)/d(Y, T) at J + cell*n*n (the pressure
+ * row and column of the reference's (nSpecie+2)^2 matrix are zero and
+ * dropped); gfc_eval_jacobian's dYdt is the derivative the Jacobian pass
+ * computes alongside (the reference's dfdx).  gfc_eval_lu_solve factorises
+ * each cell's row-major n x n matrix M + cell*n*n with the family's LU (for
+ * the workgroup family, the evaluator's factorisation variant) and solves
+ * for b ([n][ldB]) into x ([n][ldB]). */
+#define GFC_SHAPE_WAVE 1
+#define GFC_SHAPE_TEAM16 4
+#define GFC_SHAPE_WORKGROUP 512
+int gfc_eval_derivatives(gfc_evaluator* ev, int shape, int nCells, int ldY, const double* Y,
+                         double* dYdt, void* stream);
+int gfc_eval_jacobian(gfc_evaluator* ev, int shape, int nCells, int ldY, const double* Y,
+                      double* dYdt, double* J, void* stream);
+int gfc_eval_lu_solve(gfc_evaluator* ev, int shape, int nCells, int ldB, const double* M,
+                      const double* b, double* x, void* stream);
 
 /* ---- chemistry queries at the current state ----------------------------
  * The non-ODE queries of gpuFoam's chemistry model, which the reference

@@ -323,3 +323,24 @@ def test_step_underflow_is_reported():
         Yn, dt, _, st, _ = ev.solve_host(c["deltaT"], Y0, dt0, batched=batched, lane=lane)
         assert np.array_equal(st[0], ref[3][0])
         assert np.array_equal(st[1], ref[3][1]) and np.array_equal(st[2], ref[3][2])
+
+
+def test_wavefront_kernel_with_many_efficiency_sets():
+    """A 60-species mechanism with 94 distinct third-body efficiency sets
+    (n = 61: the 64-row wavefront kernel).  The kernel's LDS table of
+    third-body concentrations once held 64 sets, and the sets beyond it
+    overwrote the Gibbs tables; this solve pins the 128-set table."""
+    from gpufoam_amd.mechanism import Mechanism
+    from gpufoam_amd.synthetic import syn_states, syn_texts
+    from oracle_lib import Oracle
+    t, r = syn_texts(seed=23, nS=60, nR=300)
+    m = Mechanism(t, r, "syn60")
+    Y0, dt0 = syn_states(m, 16, seed=24)
+    c = dict(solver="Rosenbrock34", relTol=1e-6, absTol=1e-10, deltaT=1e-6)
+    ode = ode_inputs(c["solver"], relTol=c["relTol"], absTol=c["absTol"])
+    ref = Oracle(m).solve_cells(ode, Y0, c["deltaT"], dt0)
+    rho = np.linspace(0.5, 1.1, 16)
+    print("case syn60 n=16 batched, wavefront per cell", flush=True)
+    got = _device_run(m, Y0, dt0, c, ode, rho, batched=True, cells_per_wave=1)
+    from test_gpu_block import _check_conditioned
+    _check_conditioned(m, got, ref, Y0, dt0, c, ode)

@@ -263,8 +263,11 @@ def main():
         progress(f"autotuning on {args.autotune} cells")
         times = autotune(ev, c["deltaT"], dY0, ddt0, sample=args.autotune, repeats=2)
         best = times.pop("best")
+        best_lu = times.pop("best_lu", None)
         tuned = {"sample_cells": args.autotune, "best": list(best),
                  "ms": {f"{k[0]}x{k[1]}": round(v, 3) for k, v in times.items()}}
+        if best_lu is not None:
+            tuned["best_factorization"] = ["column", "panel", "mfma"][best_lu]
         progress(f"autotuner: {tuned}")
 
     def step(with_stats=False):

@@ -36,7 +36,7 @@ struct gfc_evaluator {
     int cellsPerWave = 0;  // 0: chosen automatically
     int wavesPerBlock = 0;
     int chunk = 0;
-    int factorization = GFC_LU_COLUMN; // workgroup kernel's LU variant
+    int factorization = GFC_LU_COLUMN; // workgroup kernel's LU variant (set by batched_init)
     void* batched = nullptr; // owned state of the batched launcher
     void* adjoint = nullptr; // owned state of the discrete adjoint
     gfc_mechanism mech;      // host copy of the mechanism (rate parameters can be updated)

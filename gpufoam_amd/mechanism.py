@@ -137,11 +137,17 @@ class Mechanism:
     @classmethod
     def load(cls, name: str) -> "Mechanism":
         """A bundled mechanism: 'h2' (10 species, 27 reactions), 'gri'
-        (GRI-Mech 3.0, 53 species, 325 reactions) or 'syn200' (the seeded
-        synthetic 200-species / 1200-reaction mechanism, generated)."""
+        (GRI-Mech 3.0, 53 species, 325 reactions), 'syn200' (the seeded
+        synthetic 200-species / 1200-reaction mechanism, generated) or
+        'syn100' (a 100-species / 600-reaction one of the same generator,
+        seed 7)."""
         if name == "syn200":
             from .synthetic import syn_mechanism
             return syn_mechanism()
+        if name == "syn100":
+            from .synthetic import syn_texts
+            t, r = syn_texts(seed=7, nS=100, nR=600)
+            return cls(t, r, "syn100")
         return cls.from_dir(os.path.join(MECH_DIR, name), name)
 
     # ------------------------------------------------------------- writing

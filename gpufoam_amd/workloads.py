@@ -15,6 +15,8 @@ Configurations (one CFD time step each):
                  stoichiometric CH4-air at 1100-1300 K, deltaT 1e-3
   h2_adjoint     65536 cells of the h2_small state law, deltaT 1e-4
   syn200         200 species / 1200 reactions, 32768 cells, deltaT 1e-6
+  syn100         100 species / 600 reactions (the workgroup kernel's
+                 8 x 4 register block), 16384 cells, deltaT 1e-6
 """
 from __future__ import annotations
 
@@ -39,6 +41,8 @@ CONFIGS = {
                        absTol=1e-10, deltaT=1e-4, seed=1004),
     "syn200": dict(mech="syn200", cells=32768, solver="Rosenbrock34", relTol=1e-6,
                    absTol=1e-10, deltaT=1e-6, seed=1005),
+    "syn100": dict(mech="syn100", cells=16384, solver="Rosenbrock34", relTol=1e-6,
+                   absTol=1e-10, deltaT=1e-6, seed=1006),
 }
 
 

@@ -297,8 +297,11 @@ int gfc_evaluator_get_batched_shape(const gfc_evaluator* ev, int* cellsPerWave,
  * column; GFC_LU_PANEL factorises four columns at a time in one wavefront
  * (one barrier per four columns) and applies the rank-4 trailing update with
  * vector FMAs (same arithmetic, factors bit-identical to GFC_LU_COLUMN);
- * GFC_LU_PANEL_MFMA applies that update with v_mfma_f64_16x16x4f64 (other
- * rounding of the four-term sums).  Other kernels ignore the setting. */
+ * GFC_LU_PANEL_MFMA applies that update with v_mfma_f64_16x16x4f64, which
+ * accumulates its four products in the same order (also bit-identical).
+ * The default is the measured fastest for the kernel's register block:
+ * GFC_LU_PANEL_MFMA up to nSpecie+1 = 128, GFC_LU_COLUMN above.  Other
+ * kernels ignore the setting. */
 #define GFC_LU_COLUMN 0
 #define GFC_LU_PANEL 1
 #define GFC_LU_PANEL_MFMA 2

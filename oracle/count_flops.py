@@ -34,7 +34,7 @@ L.opc_count_derivatives.argtypes = [C.POINTER(_capi.MechView), dp, u64]
 L.opc_count_jacobian.argtypes = [C.POINTER(_capi.MechView), dp, u64]
 L.opc_count_lu.argtypes = [C.c_int, dp, u64, u64]
 
-WORKLOADS = {"h2": "h2_small", "gri": "gri_production", "syn200": "syn200"}
+WORKLOADS = {"h2": "h2_small", "gri": "gri_production", "syn200": "syn200", "syn100": "syn100"}
 C_V = 40  # stage vector operations and the error norm, per component
 
 

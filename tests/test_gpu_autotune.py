@@ -47,3 +47,32 @@ def test_invalid_shapes_are_rejected():
     with pytest.raises(GfcError) as e:
         ev.set_batched_shape(4, 0)
     assert e.value.status == 1
+
+
+def test_autotune_factorization_axis_on_the_workgroup_kernel():
+    """For a mechanism on the workgroup kernel (100-species synthetic,
+    n = 101) the tuner also times the three LU variants and keeps the
+    fastest; all three give bit-identical states."""
+    from gpufoam_amd import _capi
+    from gpufoam_amd.mechanism import Mechanism
+    from gpufoam_amd.synthetic import syn_states, syn_texts
+    t, r = syn_texts(seed=7, nS=100, nR=600)
+    m = Mechanism(t, r, "syn100")
+    Y0, dt0 = syn_states(m, 64, seed=11)
+    ode = ode_inputs("Rosenbrock34", relTol=1e-6, absTol=1e-10)
+    ev = Evaluator(m, ode, 0.0, max_cells=64)
+    dY0 = torch.tensor(Y0, device="cuda")
+    ddt = torch.tensor(dt0, device="cuda")
+    print("case syn100 n=64 autotune with LU variants", flush=True)
+    times = autotune(ev, 1e-6, dY0, ddt, sample=0, repeats=1, shapes=[(1, 0)])
+    lu = {k[1]: v for k, v in times.items() if isinstance(k, tuple) and k[0] == "lu"}
+    print({k: round(v, 3) for k, v in lu.items()}, "best", times["best_lu"])
+    assert set(lu) == {_capi.LU_COLUMN, _capi.LU_PANEL, _capi.LU_PANEL_MFMA}
+    assert times["best_lu"] == min(lu, key=lu.get)
+    assert ev.factorization() == times["best_lu"]
+    outs = []
+    for v in lu:
+        ev.set_factorization(v)
+        outs.append(ev.solve_host(1e-6, Y0, dt0, batched=True)[0])
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])

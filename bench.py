@@ -204,6 +204,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--queue-chunk", type=int, choices=[0, 1], default=0,
+                    help="cells per work-queue ticket of the wavefront kernel: 0 automatic "
+                         "(16-cell chunks), 1 one cell per ticket")
     ap.add_argument("--factorization", choices=["column", "panel", "mfma"], default=None,
                     help="LU variant of the workgroup-per-cell kernel (64 < nSpecie+1 <= 208)")
     ap.add_argument("--adjoint-cells", type=int, default=65536,
@@ -253,6 +256,8 @@ def main():
     dmin = torch.zeros(1, dtype=torch.float64, device=dev)
     stats = torch.zeros((3, n), dtype=torch.int32, device=dev)
     ev = Evaluator(m, ode, 0.0, max_cells=n)
+    if args.queue_chunk:
+        ev.set_batched_shape(0, 0, args.queue_chunk)
     if args.factorization:
         ev.set_factorization({"column": _capi.LU_COLUMN, "panel": _capi.LU_PANEL,
                               "mfma": _capi.LU_PANEL_MFMA}[args.factorization])
@@ -372,7 +377,8 @@ def main():
                    "species": m.n_species, "reactions": m.n_reactions, "cells": n,
                    "solver": c["solver"], "relTol": c["relTol"], "absTol": c["absTol"],
                    "deltaT": c["deltaT"], "graph": graph,
-                   "factorization": args.factorization or "default"},
+                   "factorization": args.factorization or "default",
+                   "queue_chunk": args.queue_chunk},
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": traffic,

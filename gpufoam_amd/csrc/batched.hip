@@ -61,6 +61,19 @@ using BlockFn = void (*)(SolveArgs, int*, double*, long);
 
 constexpr int kBlockThreads = 512;
 constexpr int kLanesPerWave = 64;
+// Cells per work-queue ticket of the wavefront kernel: 16 doubles are one
+// 128-byte line of every state row, so a chunk's lines are fetched into one
+// XCD's L2 instead of into each L2 whose wavefronts take a neighbouring cell.
+constexpr int kQueueChunk = kQueueChunkMax;
+
+// Chunked tickets for all but the last kQueueChunk cells per wavefront,
+// which go out one by one so that the launch ends evenly.
+void set_queue_chunks(const gfc_evaluator* ev, SolveArgs& A, int waves) {
+    const int G = ev->chunk == 1 ? 1 : kQueueChunk;
+    const long bulk = (long)A.nCells - (long)G * waves;
+    A.qChunk = G;
+    A.qBulk = (G > 1 && bulk > 0) ? (int)(bulk / G * G) : 0;
+}
 
 BlockFn block_kernel_for(int NA, int lu) {
     static const BlockFn k8[3] = {block_solve_kernel<8, 4, 0>, block_solve_kernel<8, 4, 1>,
@@ -286,6 +299,7 @@ int batched_launch_queue(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s, i
     const int blocksNeeded = (A.nCells + perBlock - 1) / perBlock;
     int grid = pl.grid < blocksNeeded ? pl.grid : blocksNeeded;
     if (grid < 1) grid = 1;
+    if (perBlock == 1) set_queue_chunks(ev, At, grid);
     hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, s, queue);
     hipLaunchKernelGGL(pl.team, dim3(grid), dim3(64), 0, s, At, queue);
     return hipFail(hipGetLastError(), "persistent launch") ? GFC_ERR_HIP : GFC_OK;
@@ -308,6 +322,7 @@ int batched_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s,
         pl.grid = A.nCells > 0 ? A.nCells : 1;
     }
     const int grid = pl.grid;
+    if (pl.team && !t16) set_queue_chunks(ev, At, grid);
     if (graph) {
         // inside a caller's own capture the launches simply join that graph
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -472,7 +487,8 @@ int gfc_evaluator_set_batched_shape(gfc_evaluator* ev, int cellsPerWave,
                 cellsPerWave != 64) ||
         wavesPerBlock < 0 || chunk < 0 || chunk > 1) {
         gfc::setLastError("gfc_evaluator_set_batched_shape: unsupported shape "
-                          "(cellsPerWave must be 0, 1, 4 or 64, chunk 0 or 1)");
+                          "(cellsPerWave must be 0, 1, 4 or 64, chunk 0 (automatic) or 1 "
+                          "(one cell per work-queue ticket))");
         return GFC_ERR_INVALID_ARGUMENT;
     }
     if (cellsPerWave == 4) {

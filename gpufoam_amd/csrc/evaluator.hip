@@ -334,6 +334,8 @@ SolveArgs solve_args(const gfc_evaluator* ev, double deltaT, double deltaTChemMa
     A.deltaT = deltaT;
     A.deltaTChemMax = deltaTChemMax;
     A.nCells = nCells;
+    A.qChunk = 1;
+    A.qBulk = 0;
     return A;
 }
 

@@ -84,7 +84,16 @@ struct SolveArgs {
     double* dtChem;
     unsigned long long* dtMinBits; // atomicMin target (positive doubles)
     int* stats;                    // [3*ldY] or null
+    // work queue of the persistent wavefront kernel: ticket t < qBulk/qChunk
+    // is the chunk of cells [t*qChunk, (t+1)*qChunk), integrated one after
+    // the other by the wavefront that drew it; later tickets are single
+    // cells from qBulk on (qChunk = 1, qBulk = 0: one cell per ticket)
+    int qChunk;
+    int qBulk;
 };
+
+// Largest qChunk: 16 doubles are one 128-byte line of a state row.
+constexpr int kQueueChunkMax = 16;
 
 // Device-level evaluation of one kernel family's physics (the functions its
 // solve uses), one cell per wavefront / 16-lane team / workgroup:

@@ -406,6 +406,14 @@ __device__ __forceinline__ void team_lu(double (&row)[N], int n, int lane, int& 
             done = true;
             ps = j;
         }
+        {
+            // settle this step's per-lane bookkeeping here: left as select
+            // chains, every step's lane masks stay live to the end of the
+            // factorisation (SGPRs spilled into VGPR lanes)
+            int dn = done ? 1 : 0;
+            asm volatile("" : "+v"(ps), "+v"(rd), "+v"(plv), "+v"(pos), "+v"(dn));
+            done = dn != 0;
+        }
         double dj = readlane(row[j], pv);
         if (dj == 0.0) {
             dj = kSmall;
@@ -466,12 +474,34 @@ team_solve_kernel(SolveArgs A, int* queue) {
     const long ld = A.ldY;
     const RosTableau& tb = A.tab;
 
+    // the cells of one chunk share the cache lines of their state rows, so
+    // one wavefront (one XCD's L2) reads each line once
+    // The results of a chunk's cells are held back and written when the
+    // chunk ends: lane k keeps row k of every cell (the pressure in lane n),
+    // lane s the time step and counters of cell s, so each row of the chunk
+    // is written as one whole line.
     int cell = queue ? 0 : blockIdx.x;
+    int cBase = cell, cEnd = cell + 1;
+    double yb[kQueueChunkMax];
+#pragma unroll
+    for (int i = 0; i < kQueueChunkMax; ++i) yb[i] = 0.0;
+    double dtb = 0.0;
+    int sb0 = 0, sb1 = 0, sb2 = 0;
+    if (queue) cEnd = 0;
     for (;;) {
-        if (queue) {
-            int c = 0;
-            if (lane == 0) c = atomicAdd(queue, 1);
-            cell = __builtin_amdgcn_readfirstlane(c);
+        if (queue && ++cell >= cEnd) {
+            int t = 0;
+            if (lane == 0) t = atomicAdd(queue, 1);
+            t = __builtin_amdgcn_readfirstlane(t);
+            const int nChunks = A.qBulk / A.qChunk;
+            if (t < nChunks) {
+                cell = t * A.qChunk;
+                cEnd = cell + A.qChunk;
+            } else {
+                cell = A.qBulk + (t - nChunks);
+                cEnd = cell + 1;
+            }
+            cBase = cell;
         }
         if (cell >= A.nCells) break;
 
@@ -598,17 +628,32 @@ team_solve_kernel(SolveArgs A, int* queue) {
         }
         if (A.RR && lane < nS)
             A.RR[lane * ld + cell] = A.rho[cell] * (y - y0cell) / A.deltaT;
-        if (A.Ynew) {
-            if (lane < n) A.Ynew[lane * ld + cell] = y;
-            if (lane == n) A.Ynew[(long)n * ld + cell] = p;
+        if (lane == 0 && A.dtMinBits) atomic_min_positive(A.dtMinBits, dtChem);
+        const int slot = cell - cBase;
+        const double yOut = (lane < n) ? y : p;
+#pragma unroll
+        for (int i = 0; i < kQueueChunkMax; ++i) yb[i] = (i == slot) ? yOut : yb[i];
+        if (lane == slot) {
+            dtb = fmin(dtChem, A.deltaTChemMax);
+            sb0 = status;
+            sb1 = attempts;
+            sb2 = accepted;
         }
-        if (lane == 0) {
-            if (A.dtMinBits) atomic_min_positive(A.dtMinBits, dtChem);
-            A.dtChem[cell] = fmin(dtChem, A.deltaTChemMax);
-            if (A.stats) {
-                A.stats[cell] = status;
-                A.stats[ld + cell] = attempts;
-                A.stats[2 * ld + cell] = accepted;
+        if (cell + 1 >= cEnd || cell + 1 >= A.nCells) {
+            const int cnt = min(cEnd, A.nCells) - cBase;
+            if (A.Ynew && lane <= n) {
+                double* out = A.Ynew + (long)lane * ld + cBase;
+#pragma unroll
+                for (int i = 0; i < kQueueChunkMax; ++i)
+                    if (i < cnt) out[i] = yb[i];
+            }
+            if (lane < cnt) {
+                A.dtChem[cBase + lane] = dtb;
+                if (A.stats) {
+                    A.stats[cBase + lane] = sb0;
+                    A.stats[ld + cBase + lane] = sb1;
+                    A.stats[2 * ld + cBase + lane] = sb2;
+                }
             }
         }
         if (!queue) break;

@@ -283,8 +283,10 @@ int gfc_solve_noalloc(gfc_evaluator* ev, double deltaT, double deltaTChemMax,
  * third-body efficiency sets, at most 3 species per reaction side; other
  * mechanisms return GFC_ERR_INVALID_ARGUMENT), 64 = one lane per cell.
  * wavesPerBlock: resident wavefronts per compute unit of the persistent
- * grid (0 = as many as fit).  chunk: cells taken per queue grab (0 or 1;
- * larger values are reserved). */
+ * grid (0 = as many as fit).  chunk: cells per work-queue ticket of the
+ * wavefront kernel: 0 = automatic (16 consecutive cells, one 128-byte line
+ * of every state row, per ticket, except the last 16 per wavefront, which
+ * go out one at a time), 1 = one cell per ticket. */
 int gfc_evaluator_set_batched_shape(gfc_evaluator* ev, int cellsPerWave,
                                     int wavesPerBlock, int chunk);
 int gfc_evaluator_get_batched_shape(const gfc_evaluator* ev, int* cellsPerWave,

@@ -125,6 +125,8 @@ def lib() -> C.CDLL:
             "gfc_evaluator_set_rate_parameters": (C.c_int, [vp, _dp]),
             "gfc_adjoint_set_checkpoint_interval": (C.c_int, [vp, C.c_int]),
             "gfc_adjoint_checkpoint_interval": (C.c_int, [vp]),
+            "gfc_adjoint_workspace_bytes": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int]),
+            "gfc_adjoint_plan_interval": (C.c_int, [C.c_double, C.c_int, C.c_int, C.c_int]),
             "gfc_adjoint_forward": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, vp, vp, vp,
                                               vp, vp]),
             "gfc_adjoint_backward": (C.c_int, [vp, vp, C.c_int, vp, vp, vp]),
@@ -163,6 +165,7 @@ def exported_symbols() -> list[str]:
         "gfc_eval_lu_solve", "gfc_evaluator_param_count",
         "gfc_evaluator_get_rate_parameters", "gfc_evaluator_set_rate_parameters",
         "gfc_adjoint_set_checkpoint_interval", "gfc_adjoint_checkpoint_interval",
+        "gfc_adjoint_workspace_bytes", "gfc_adjoint_plan_interval",
         "gfc_adjoint_forward", "gfc_adjoint_backward", "gfc_adjoint_replay",
         "gfc_calculate", "gfc_tc", "gfc_reaction_rr", "gfc_specie_reaction_rr", "gfc_qdot",
         "gfc_chemistry_switches_from_foam", "gfc_solve_noalloc",

@@ -17,8 +17,8 @@
 //
 // Memory: step sizes maxSteps x cells, checkpoints ceil(maxSteps/K) x n x
 // cells, per-cell scratch (forward + adjoint arrays + a K-state segment
-// buffer).  K is chosen automatically so that the checkpoints fit in
-// about 40 % of the free device memory, or set by the caller.
+// buffer).  K is chosen automatically as the smallest interval whose whole
+// workspace fits in 80 % of the free device memory, or set by the caller.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -209,6 +209,33 @@ static AdjLayout layout(gfc_evaluator* ev) {
     return W;
 }
 
+// Bytes of the adjoint workspace for `cap` cells at checkpoint interval K:
+// per cell the forward and adjoint scratch plus the K-state segment buffer,
+// ceil(maxSteps/K) checkpoints of n, maxSteps step sizes, the initial state
+// (n+1), the final state (n) and the accepted-step count.
+double adjoint_workspace_bytes(long cap, long n, long maxSteps, long K, long laneD,
+                               long adjD) {
+    const long nck = (maxSteps + K - 1) / K;
+    const double perCell = 8.0 * (double)(laneD + adjD + K * n + nck * n + maxSteps +
+                                          (n + 1) + n) + 4.0;
+    return perCell * (double)cap + sizeof(int);
+}
+
+// Automatic checkpoint interval: the smallest K whose whole workspace fits
+// in `budget` bytes.  The total falls with K while the checkpoints dominate
+// and rises again once the segment buffer does; its minimum lies at
+// K = sqrt(maxSteps), so the search stops at ceil(sqrt(maxSteps)) and
+// returns that K when nothing fits (the allocation then reports the
+// shortage).
+long adjoint_plan_interval(double budget, long cap, long n, long maxSteps, long laneD,
+                           long adjD) {
+    maxSteps = std::max(1L, maxSteps);
+    const long kMax = std::min(maxSteps, (long)std::ceil(std::sqrt((double)maxSteps)));
+    for (long K = 1; K <= kMax; ++K)
+        if (adjoint_workspace_bytes(cap, n, maxSteps, K, laneD, adjD) <= budget) return K;
+    return kMax;
+}
+
 // Sizes (and allocates) the workspace for nCells cells.
 static int adjoint_reserve(gfc_evaluator* ev, int nCells) {
     AdjState* st = state(ev);
@@ -216,21 +243,19 @@ static int adjoint_reserve(gfc_evaluator* ev, int nCells) {
     const long cap = ((long)nCells + 63) / 64 * 64;
     const long maxSteps = std::max(1, ev->ode.maxSteps);
     int K = st->Kreq;
+    const long laneD = lane::lane_scratch_doubles(ev->nS);
+    const long adjD = adj::adj_scratch_doubles(ev->nS);
     if (K <= 0) {
         size_t freeB = 0, totalB = 0;
         if (hipFail(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo")) return GFC_ERR_HIP;
-        freeB += (size_t)st->cap * 8 *
-                 ((st->laneD + st->adjD + (long)st->K * n) + maxSteps + n + 2 * n + 2);
-        if (st->ck) freeB += (size_t)8 * ((maxSteps + st->K - 1) / std::max(1, st->K)) * n * st->cap;
-        const double budget = 0.4 * (double)freeB;
-        const double full = 8.0 * (double)maxSteps * n * cap;
-        K = full <= budget ? 1 : (int)std::ceil(full / budget);
-        K = std::min<long>(K, maxSteps);
+        if (st->scratch) freeB += adjoint_workspace_bytes(st->cap, n, st->maxSteps, st->K,
+                                                          st->laneD, st->adjD);
+        K = (int)adjoint_plan_interval(0.8 * (double)freeB, cap, n, maxSteps, laneD, adjD);
     }
     if (st->cap >= cap && st->K == K && st->scratch) return GFC_OK;
     st->release();
-    st->laneD = lane::lane_scratch_doubles(ev->nS);
-    st->adjD = adj::adj_scratch_doubles(ev->nS);
+    st->laneD = laneD;
+    st->adjD = adjD;
     st->K = K;
     st->cap = cap;
     st->maxSteps = maxSteps;
@@ -337,6 +362,22 @@ int gfc_adjoint_set_checkpoint_interval(gfc_evaluator* ev, int K) {
 int gfc_adjoint_checkpoint_interval(const gfc_evaluator* ev) {
     if (!ev || !ev->adjoint) return 0;
     return static_cast<const AdjState*>(ev->adjoint)->K;
+}
+
+double gfc_adjoint_workspace_bytes(int nCells, int nSpecie, int maxSteps, int K) {
+    if (nCells < 0 || nSpecie <= 0 || K <= 0) return -1.0;
+    const long cap = ((long)nCells + 63) / 64 * 64;
+    return adjoint_workspace_bytes(cap, nSpecie + 1, std::max(1, maxSteps), K,
+                                   lane::lane_scratch_doubles(nSpecie),
+                                   adj::adj_scratch_doubles(nSpecie));
+}
+
+int gfc_adjoint_plan_interval(double budgetBytes, int nCells, int nSpecie, int maxSteps) {
+    if (nCells < 0 || nSpecie <= 0) return 0;
+    const long cap = ((long)nCells + 63) / 64 * 64;
+    return (int)adjoint_plan_interval(budgetBytes, cap, nSpecie + 1, maxSteps,
+                                      lane::lane_scratch_doubles(nSpecie),
+                                      adj::adj_scratch_doubles(nSpecie));
 }
 
 int gfc_adjoint_forward(gfc_evaluator* ev, double deltaT, int nCells, int ldY,

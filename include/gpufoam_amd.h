@@ -422,6 +422,15 @@ int gfc_evaluator_set_rate_parameters(gfc_evaluator* ev, const double* params);
  * All three are asynchronous on `stream`. */
 int gfc_adjoint_set_checkpoint_interval(gfc_evaluator* ev, int K); /* 0: auto */
 int gfc_adjoint_checkpoint_interval(const gfc_evaluator* ev);
+/* Workspace planning (host only, no device call; no counterpart in gpuFoam):
+ * the bytes gfc_adjoint_forward allocates for nCells cells of an nSpecie
+ * mechanism at checkpoint interval K, and the interval the automatic choice
+ * takes for a budget: the smallest K whose whole workspace fits, searched up
+ * to ceil(sqrt(maxSteps)), where the workspace is smallest (returned when
+ * nothing fits).  The automatic choice uses 80 % of the free device memory
+ * (plus the workspace already held) as the budget. */
+double gfc_adjoint_workspace_bytes(int nCells, int nSpecie, int maxSteps, int K);
+int gfc_adjoint_plan_interval(double budgetBytes, int nCells, int nSpecie, int maxSteps);
 int gfc_adjoint_forward(gfc_evaluator* ev, double deltaT, int nCells, int ldY,
                         const double* Y0, double* deltaTChem, double* Ynew,
                         int* cellStats, void* stream);

@@ -36,6 +36,7 @@ def lib() -> C.CDLL:
             "adjh_solve_T": [C.c_int, _dp, _dp],
             "adjh_cell": [_VP, _OP, C.c_double, C.c_double, _dp, _dp, C.c_int, _dp, _dp, _dp],
             "adjh_replay": [_VP, _OP, _dp, C.c_int, _dp],
+            "adjh_cell_solve": [_VP, _OP, C.c_double, _dp, _dp, C.POINTER(C.c_int)],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)
@@ -110,3 +111,14 @@ def replay(mech, ode, y0, hs) -> np.ndarray:
     h = np.ascontiguousarray(hs, dtype=np.float64)
     lib().adjh_replay(C.pointer(mech.view), C.byref(ode), P(z), len(h), P(h))
     return z
+
+
+def cell_solve(mech, ode, y0, dtChem, deltaT):
+    """One cell through the library's one-lane solver built for the host.
+    Returns (final state, dtChem, [status, attempts, accepted])."""
+    y = np.array(y0, dtype=np.float64)
+    dt = np.array([dtChem], dtype=np.float64)
+    st = np.zeros(3, dtype=np.int32)
+    lib().adjh_cell_solve(C.pointer(mech.view), C.byref(ode), deltaT, P(y), P(dt),
+                          st.ctypes.data_as(C.POINTER(C.c_int)))
+    return y, dt[0], st

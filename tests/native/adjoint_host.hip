@@ -197,6 +197,35 @@ int adjh_cell(const gfc_mech_view* v, const gfc_ode_inputs* ode, double Treact,
     return rec.count;
 }
 
+// One cell through the one-lane solver of the library (lane_solve_kernel's
+// cell loop: singleCellSolver.H:27-51 with the status bits of the
+// library).  y: [nS+2] in/out; stats: status, attempts, accepted.
+void adjh_cell_solve(const gfc_mech_view* v, const gfc_ode_inputs* ode, double deltaT,
+                     double* y, double* dtChem, int* stats) {
+    HostMech hm(v);
+    const int nS = v->n_species, n = nS + 1;
+    Work w(nS);
+    SolveArgs A = args_for(hm.d, ode);
+    A.deltaT = deltaT;
+    const lane::LaneBuf b = w.a.b;
+    copy_in(b.y(), y, n);
+    const double p = y[nS + 1];
+    int status = 0, attempts = 0, accepted = 0;
+    lane::NoRecord rec;
+    double timeLeft = deltaT;
+    while (timeLeft > kSmall) {
+        const double dt = timeLeft;
+        status |= lane::ode_solve(A, p, dt, *dtChem, b, attempts, accepted, rec);
+        for (int i = 0; i < nS; i++) b.y()[i] = std::fmax(0.0, b.y()[i]);
+        timeLeft -= dt;
+        if (status & GFC_CELL_STEP_UNDERFLOW) break;
+    }
+    for (int k = 0; k < n; ++k) y[k] = b.y()[k];
+    stats[0] = status;
+    stats[1] = attempts;
+    stats[2] = accepted;
+}
+
 // Re-integrates N recorded steps from y (in/out, [nS+2]) and clamps.
 void adjh_replay(const gfc_mech_view* v, const gfc_ode_inputs* ode, double* y, int N,
                  const double* hs) {

@@ -184,3 +184,30 @@ def test_skipped_cell_gradient_is_identity():
     w = np.random.default_rng(8).standard_normal(n)
     _, gy, gth, _, N = AH.cell_adjoint(m, ode, y, dt0, c["deltaT"], w, Treact=1e4)
     assert N == 0 and np.array_equal(gy, w) and not gth.any()
+
+
+@pytest.mark.parametrize("cells,nS,max_steps", [(65536, 10, 10000), (262144, 53, 10000),
+                                                (32768, 200, 10000), (1000, 10, 1)])
+def test_checkpoint_interval_sizes_the_whole_workspace(cells, nS, max_steps):
+    """The automatic interval is the smallest K whose whole workspace
+    (checkpoints, the K-state segment buffer, step sizes, per-cell scratch)
+    fits the budget; the workspace is smallest near sqrt(maxSteps), which
+    caps the search; nothing fitting returns that cap."""
+    from gpufoam_amd import _capi
+    L = _capi.lib()
+    ws = lambda K: L.gfc_adjoint_workspace_bytes(cells, nS, max_steps, K)
+    kcap = min(max_steps, int(np.ceil(np.sqrt(max_steps))))
+    # the total falls while the checkpoints dominate, up to the cap
+    sizes = [ws(K) for K in range(1, kcap + 1)]
+    assert all(b <= a * (1 + 1e-12) for a, b in zip(sizes, sizes[1:]))
+    assert ws(1) >= 8.0 * max_steps * (nS + 1) * cells  # checkpoints alone
+    for frac in (1.5, 1.0, 0.6, 0.3):
+        budget = frac * ws(1)
+        K = L.gfc_adjoint_plan_interval(budget, cells, nS, max_steps)
+        assert 1 <= K <= kcap
+        if ws(K) <= budget:
+            assert K == 1 or ws(K - 1) > budget
+        else:  # nothing fits: the smallest workspace
+            assert K == kcap and all(ws(k) > budget for k in range(1, kcap + 1))
+    assert L.gfc_adjoint_plan_interval(1e300, cells, nS, max_steps) == 1
+    assert L.gfc_adjoint_plan_interval(0.0, cells, nS, max_steps) == kcap

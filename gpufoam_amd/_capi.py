@@ -40,7 +40,7 @@ LANE_PER_CELL = 0x8
 # factorisation of the workgroup-per-cell kernel
 LU_COLUMN, LU_PANEL, LU_PANEL_MFMA = 0, 1, 2
 # kernel families of the device-level evaluations
-SHAPE_AUTO, SHAPE_WAVE, SHAPE_TEAM16, SHAPE_WORKGROUP = 0, 1, 4, 512
+SHAPE_AUTO, SHAPE_WAVE, SHAPE_PAIR, SHAPE_TEAM16, SHAPE_WORKGROUP = 0, 1, 2, 4, 512
 
 
 class GfcError(RuntimeError):

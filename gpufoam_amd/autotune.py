@@ -2,8 +2,9 @@
 
 The search space is the evaluator's batched shapes (gfc_evaluator_set_batched_shape):
 one wavefront per cell with 1, 2, 4, ... resident wavefronts per compute
-unit up to what fits, four cells per wavefront (16-lane teams, for
-mechanisms with nSpecie+1 <= 16), and one lane per cell; for mechanisms that
+unit up to what fits, one cell per two wavefronts (32 < nSpecie+1 <= 64),
+four cells per wavefront (16-lane teams, for mechanisms with
+nSpecie+1 <= 16), and one lane per cell; for mechanisms that
 run on the workgroup-per-cell kernel (64 < nSpecie+1 <= 208) also its LU
 variant (gfc_evaluator_set_factorization: column by column, four-column
 panels with vector-FMA or MFMA trailing updates).  Each candidate is timed
@@ -35,7 +36,8 @@ def autotune(ev, deltaT: float, Y0, dtChem, sample: int = 4096, repeats: int = 2
         Ys, dts = Y0[:, :n].contiguous(), dtChem.clone()
     Yn = torch.empty_like(Ys)
     if shapes is None:
-        shapes = [(1, w) for w in (1, 2, 4, 8, 16)] + [(1, 0), (4, 0), (64, 0)]
+        shapes = ([(1, w) for w in (1, 2, 4, 8, 16)] +
+                  [(1, 0), (2, 0), (4, 0), (64, 0)])
     from ._capi import LU_COLUMN, LU_PANEL, LU_PANEL_MFMA, GfcError
 
     def timed():
@@ -58,7 +60,8 @@ def autotune(ev, deltaT: float, Y0, dtChem, sample: int = 4096, repeats: int = 2
         try:
             ev.set_batched_shape(cpw, wpc)
         except GfcError:
-            continue  # a shape this mechanism cannot run (4 cells need n <= 16)
+            continue  # a shape this mechanism cannot run (4 cells need n <= 16,
+            # two wavefronts per cell 32 < n <= 64)
         times[(cpw, wpc)] = timed()
     best = min(times, key=times.get)
     ev.set_batched_shape(*best)

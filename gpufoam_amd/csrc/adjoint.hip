@@ -40,6 +40,14 @@ template <int N>
 __global__ void team16_adjoint_reverse_kernel(SolveArgs A, t16::T16Record R, double* seg,
                                               long segPer, const double* Ybar, long ldb,
                                               double* Y0bar, double* thBar, int* queue);
+// One wavefront per cell (team_kernel.hip, wave_adjoint.hip): used for
+// 16 < nSpecie+1 <= 64 unless one lane per cell is asked for.
+template <int N>
+__global__ void team_record_kernel(SolveArgs A, int* queue, t16::T16Record R);
+template <int N>
+__global__ void wave_adjoint_reverse_kernel(SolveArgs A, t16::T16Record R, double* seg,
+                                            long segPer, const double* Ybar, long ldb,
+                                            double* Y0bar, double* thBar, int* queue);
 
 namespace {
 
@@ -68,6 +76,7 @@ struct AdjState {
     double deltaT = 0;
     bool valid = false;
     int teamN = 0;        // > 0: recorded by the four-cells-per-wavefront kernel
+    int waveN = 0;        // > 0: recorded by the wavefront-per-cell kernel
     int numCUs = 0;
     void release() {
         void* ptrs[] = {scratch, ck, hs, y0, yEnd, nAcc, queue};
@@ -279,6 +288,34 @@ static t16::T16Record record_of(const AdjState* st, const AdjLayout& W) {
                           W.nck, W.maxSteps, W.K};
 }
 
+// The wavefront-per-cell form (row capacity N) when the evaluator's shape
+// leaves the choice to the mechanism (automatic, one or two wavefronts per
+// cell) and 16 < nSpecie+1 <= 64; 0 otherwise.
+static int wave_adjoint_rows(const gfc_evaluator* ev) {
+    if (ev->cellsPerWave != 0 && ev->cellsPerWave != 1 && ev->cellsPerWave != 2) return 0;
+    if (ev->nS + 1 <= 16) return 0;
+    return wave_kernel_rows(ev);
+}
+
+// Persistent grid of a kernel of one wavefront per `perWave` cells: the
+// wavefronts that stay resident, at most one per perWave cells.
+template <class K>
+static int resident_grid(AdjState* st, int device, K kernel, int nCells, int perWave) {
+    if (st->numCUs == 0) {
+        if (hipDeviceGetAttribute(&st->numCUs, hipDeviceAttributeMultiprocessorCount, device) !=
+                hipSuccess ||
+            st->numCUs <= 0)
+            st->numCUs = 1;
+    }
+    int perCU = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void*)kernel, 64, 0) !=
+            hipSuccess ||
+        perCU <= 0)
+        perCU = 1;
+    const long waves = ((long)nCells + perWave - 1) / perWave;
+    return (int)std::max(1L, std::min<long>((long)st->numCUs * perCU, waves));
+}
+
 // Persistent grid of a four-cells-per-wavefront kernel: the wavefronts that
 // stay resident, at most one per four cells.
 template <class K>
@@ -416,7 +453,19 @@ int gfc_adjoint_forward(gfc_evaluator* ev, double deltaT, int nCells, int ldY,
     A.stats = cellStats;
     int N = 0;
     st->teamN = team16_selected(ev, nCells, N) ? N : 0;
-    if (st->teamN) {
+    st->waveN = st->teamN ? 0 : wave_adjoint_rows(ev);
+    if (st->waveN) {
+        SolveArgs At = A;
+        At.m = ev->dmTeam;
+        At.qChunk = 1;  // one cell per work-queue ticket
+        At.qBulk = 0;
+        const t16::T16Record R = record_of(st, layout(ev));
+        auto k = st->waveN == 32 ? team_record_kernel<32>
+               : st->waveN == 54 ? team_record_kernel<54> : team_record_kernel<64>;
+        hipLaunchKernelGGL(adj_reset_queue_kernel, dim3(1), dim3(1), 0, s, st->queue);
+        hipLaunchKernelGGL(k, dim3(resident_grid(st, ev->device, k, nCells, 1)), dim3(64), 0,
+                           s, At, st->queue, R);
+    } else if (st->teamN) {
         SolveArgs At = A;
         At.m = ev->dmTeam;
         const t16::T16Record R = record_of(st, layout(ev));
@@ -451,7 +500,18 @@ int gfc_adjoint_backward(gfc_evaluator* ev, const double* Ybar, int ldb, double*
     hipStream_t s = static_cast<hipStream_t>(stream);
     SolveArgs A = solve_args(ev, st->deltaT, 1e300, st->nCells);
     const AdjLayout W = layout(ev);
-    if (st->teamN) {
+    if (st->waveN) {
+        SolveArgs At = A;
+        At.m = ev->dmTeam;
+        auto k = st->waveN == 32 ? wave_adjoint_reverse_kernel<32>
+               : st->waveN == 54 ? wave_adjoint_reverse_kernel<54>
+                                 : wave_adjoint_reverse_kernel<64>;
+        double* seg = st->scratch + (W.laneD + W.adjD) * lane::kSlotStride;
+        hipLaunchKernelGGL(adj_reset_queue_kernel, dim3(1), dim3(1), 0, s, st->queue);
+        hipLaunchKernelGGL(k, dim3(resident_grid(st, ev->device, k, st->nCells, 1)), dim3(64), 0,
+                           s, At, record_of(st, W), seg, W.perScratch, Ybar, (long)ldb, Y0bar,
+                           paramBar, st->queue);
+    } else if (st->teamN) {
         SolveArgs At = A;
         At.m = ev->dmTeam;
         auto k = (st->teamN == 11) ? team16_adjoint_reverse_kernel<11>

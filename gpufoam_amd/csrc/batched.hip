@@ -10,8 +10,9 @@
 //    arguments; changed arguments update the graph in place.  Called inside
 //    the caller's own stream capture, the launches join that graph instead.
 //  * Launch shape (autotuned by gpufoam_amd/autotune.py): cells per
-//    wavefront 1 (the wavefront kernel) or 64 (one lane per cell), and the
-//    number of wavefronts kept resident per compute unit.
+//    wavefront 1 (the wavefront kernel), 4 (16-lane teams), 64 (one lane
+//    per cell) or code 2 (two wavefronts per cell), and the number of
+//    workgroups kept resident per compute unit.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -25,6 +26,10 @@ template <int N>
 __global__ void team_solve_kernel(SolveArgs A, int* queue);
 template <int N>
 __global__ void team16_solve_kernel(SolveArgs A, int* queue);
+template <int N, bool EXACT>
+__global__ void pair_solve_kernel(SolveArgs A, int* queue);
+template <int N>
+__global__ void pair_eval_kernel(EvalArgs E);
 template <int NA, int NC, int LUV>
 __global__ void block_solve_kernel(SolveArgs A, int* queue, double* scratch, long perWG);
 __global__ void lane_solve_kernel(SolveArgs A, double* scratch, long per);
@@ -92,7 +97,8 @@ struct Plan {
     KernelFn team = nullptr;
     BlockFn blk = nullptr;
     int grid = 0;
-    int key = 0; // identifies the kernel for graph reuse
+    int threads = 64; // workgroup size of `team` (128: two wavefronts per cell)
+    int key = 0;      // identifies the kernel for graph reuse
 };
 
 // Row capacity of the wavefront kernel for n = nSpecie+1: exact sizes for
@@ -114,6 +120,18 @@ KernelFn team16_kernel_for(const DeviceMech& m, int& N) {
     if (m.nSets > kSetsT || m.teamWidth > kTeamWidth) return nullptr;
     if (n == 11) { N = 11; return team16_solve_kernel<11>; }
     if (n <= 16) { N = 16; return team16_solve_kernel<16>; }
+    return nullptr;
+}
+
+// Two wavefronts per cell: 32 < n <= 64, at most kSetsT efficiency sets (the
+// dense third-body part is fused into the row transform), at most
+// kTeamWidth species per reaction side.
+KernelFn pair_kernel_for(const DeviceMech& m, int& N) {
+    const int n = m.nS + 1;
+    N = 0;
+    if (m.nSets > kSetsT || m.teamWidth > kTeamWidth || m.extended) return nullptr;
+    if (n == 54) { N = 54; return pair_solve_kernel<54, true>; }
+    if (n > 32 && n <= 64) { N = 64; return pair_solve_kernel<64, false>; }
     return nullptr;
 }
 
@@ -181,10 +199,10 @@ void batched_free(gfc_evaluator* ev) {
     ev->batched = nullptr;
 }
 
-// Wavefronts per CU the wavefront kernel can keep resident.
-static int resident_blocks(KernelFn fn) {
+// Workgroups of `threads` per CU the kernel can keep resident.
+static int resident_blocks(KernelFn fn, int threads = 64) {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)fn, 64, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)fn, threads, 0) !=
             hipSuccess ||
         blocks <= 0)
         blocks = 1;
@@ -197,7 +215,7 @@ static int enqueue(BatchedState* b, const SolveArgs& A, hipStream_t s, const Pla
         hipLaunchKernelGGL(pl.blk, dim3(pl.grid), dim3(kBlockThreads), 0, s, A, b->queue,
                            b->blkScratch, b->blkPerWG);
     else
-        hipLaunchKernelGGL(pl.team, dim3(pl.grid), dim3(64), 0, s, A, b->queue);
+        hipLaunchKernelGGL(pl.team, dim3(pl.grid), dim3(pl.threads), 0, s, A, b->queue);
     if (hipFail(hipGetLastError(), "persistent launch")) return GFC_ERR_HIP;
     return GFC_OK;
 }
@@ -210,6 +228,15 @@ constexpr int kTeam16MinCells = 16384;
 // batched solve takes 0.39x the time of the one-cell form at 65536 H2 cells,
 // 0.41x to 0.90x at 16384 and 1.13x at 4096, where 1024 wavefronts leave
 // most SIMDs idle.
+int wave_kernel_rows(const gfc_evaluator* ev) {
+    int N = 0;
+    const DeviceMech& m = ev->dmTeam;
+    if (m.extended || m.teamWidth > kTeamWidth || m.nSets > kTeamMaxSets ||
+        !team_kernel_for(m.nS + 1, N))
+        return 0;
+    return N;
+}
+
 bool team16_selected(const gfc_evaluator* ev, int nCells, int& N) {
     const bool want = ev->cellsPerWave == 4 ||
                       (ev->cellsPerWave == 0 && nCells >= kTeam16MinCells);
@@ -228,11 +255,19 @@ static void choose_plan(gfc_evaluator* ev, int nCells, Plan& pl, int& cellsPerBl
     // beyond gpuFoam's four (the wavefront and workgroup kernels evaluate
     // those four from their packed records)
     if (ev->cellsPerWave == 64 || ev->dmTeam.extended) return;
-    int N16 = 0, N = 0;
+    int N16 = 0, N = 0, N2 = 0;
     KernelFn t16 = team16_selected(ev, nCells, N16) ? team16_kernel_for(ev->dmTeam, N16)
                                                     : nullptr;
     KernelFn team = team_kernel_for(n, N);
-    if (t16) {
+    KernelFn pair = ev->cellsPerWave == 2 ? pair_kernel_for(ev->dmTeam, N2) : nullptr;
+    if (pair) {
+        int perCU = resident_blocks(pair, 2 * kLanesPerWave);
+        if (ev->wavesPerBlock > 0 && ev->wavesPerBlock < perCU) perCU = ev->wavesPerBlock;
+        pl.team = pair;
+        pl.threads = 2 * kLanesPerWave;
+        pl.grid = b->numCUs * perCU;
+        pl.key = 3000 + N2;
+    } else if (t16) {
         int perCU = resident_blocks(t16);
         if (ev->wavesPerBlock > 0 && ev->wavesPerBlock < perCU) perCU = ev->wavesPerBlock;
         pl.team = t16;
@@ -279,7 +314,7 @@ int plain_launch(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s, bool lane
         // one wavefront per cell (or per four cells): the grid covers the
         // cells and the hardware dispatcher fills the CUs
         const int grid = (A.nCells + perBlock - 1) / perBlock;
-        hipLaunchKernelGGL(pl.team, dim3(grid), dim3(64), 0, s, At, nullptr);
+        hipLaunchKernelGGL(pl.team, dim3(grid), dim3(pl.threads), 0, s, At, nullptr);
     }
     return hipFail(hipGetLastError(), "drop-in launch") ? GFC_ERR_HIP : GFC_OK;
 }
@@ -301,7 +336,7 @@ int batched_launch_queue(gfc_evaluator* ev, const SolveArgs& A, hipStream_t s, i
     if (grid < 1) grid = 1;
     if (perBlock == 1) set_queue_chunks(ev, At, grid);
     hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, s, queue);
-    hipLaunchKernelGGL(pl.team, dim3(grid), dim3(64), 0, s, At, queue);
+    hipLaunchKernelGGL(pl.team, dim3(grid), dim3(pl.threads), 0, s, At, queue);
     return hipFail(hipGetLastError(), "persistent launch") ? GFC_ERR_HIP : GFC_OK;
 }
 
@@ -389,7 +424,7 @@ int device_eval(gfc_evaluator* ev, int shape, EvalArgs E, hipStream_t s) {
         Plan pl;
         int perBlock = 1;
         choose_plan(ev, ev->maxCells, pl, perBlock);
-        shape = pl.blk ? 512 : perBlock > 1 ? 4 : pl.team ? 1 : 64;
+        shape = pl.blk ? 512 : perBlock > 1 ? 4 : pl.threads == 128 ? 2 : pl.team ? 1 : 64;
     }
     E.m = ev->dmTeam;
     if (E.nCells == 0) return GFC_OK;
@@ -405,6 +440,16 @@ int device_eval(gfc_evaluator* ev, int shape, EvalArgs E, hipStream_t s) {
         void (*k)(EvalArgs) = N == 11 ? team_eval_kernel<11> : N == 32 ? team_eval_kernel<32>
                             : N == 54 ? team_eval_kernel<54> : team_eval_kernel<64>;
         hipLaunchKernelGGL(k, dim3(E.nCells), dim3(64), 0, s, E);
+    } else if (shape == 2) {
+        int N = 0;
+        if (!pair_kernel_for(ev->dmTeam, N)) {
+            setLastError("device evaluation: no two-wavefront kernel for this mechanism "
+                         "(32 < nSpecie+1 <= 64, at most 16 efficiency sets, 3 species per "
+                         "side)");
+            return GFC_ERR_UNSUPPORTED;
+        }
+        void (*k)(EvalArgs) = N == 54 ? pair_eval_kernel<54> : pair_eval_kernel<64>;
+        hipLaunchKernelGGL(k, dim3(E.nCells), dim3(2 * kLanesPerWave), 0, s, E);
     } else if (shape == 4) {
         int N = 0;
         if (!team16_kernel_for(ev->dmTeam, N)) {
@@ -426,7 +471,7 @@ int device_eval(gfc_evaluator* ev, int shape, EvalArgs E, hipStream_t s) {
         hipLaunchKernelGGL(k, dim3(grid), dim3(kBlockThreads), 0, s, E, b->blkScratch,
                            b->blkPerWG);
     } else {
-        setLastError("device evaluation: shape must be 0, 1, 4 or 512");
+        setLastError("device evaluation: shape must be 0, 1, 2, 4 or 512");
         return GFC_ERR_INVALID_ARGUMENT;
     }
     return hipFail(hipGetLastError(), "device evaluation launch") ? GFC_ERR_HIP : GFC_OK;
@@ -483,13 +528,22 @@ int gfc_eval_lu_solve(gfc_evaluator* ev, int shape, int nCells, int ldB, const d
 
 int gfc_evaluator_set_batched_shape(gfc_evaluator* ev, int cellsPerWave,
                                     int wavesPerBlock, int chunk) {
-    if (!ev || (cellsPerWave != 0 && cellsPerWave != 1 && cellsPerWave != 4 &&
-                cellsPerWave != 64) ||
+    if (!ev || (cellsPerWave != 0 && cellsPerWave != 1 && cellsPerWave != 2 &&
+                cellsPerWave != 4 && cellsPerWave != 64) ||
         wavesPerBlock < 0 || chunk < 0 || chunk > 1) {
         gfc::setLastError("gfc_evaluator_set_batched_shape: unsupported shape "
-                          "(cellsPerWave must be 0, 1, 4 or 64, chunk 0 (automatic) or 1 "
-                          "(one cell per work-queue ticket))");
+                          "(cellsPerWave must be 0, 1, 2 (two wavefronts per cell), 4 or "
+                          "64, chunk 0 (automatic) or 1 (one cell per work-queue ticket))");
         return GFC_ERR_INVALID_ARGUMENT;
+    }
+    if (cellsPerWave == 2) {
+        int N = 0;
+        if (!gfc::pair_kernel_for(ev->dmTeam, N)) {
+            gfc::setLastError("gfc_evaluator_set_batched_shape: two wavefronts per cell "
+                              "needs 32 < nSpecie+1 <= 64, at most 16 third-body "
+                              "efficiency sets and at most 3 species per reaction side");
+            return GFC_ERR_INVALID_ARGUMENT;
+        }
     }
     if (cellsPerWave == 4) {
         int N = 0;

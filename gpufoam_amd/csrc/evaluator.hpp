@@ -66,4 +66,8 @@ int noalloc_init_device(int device);
 // Whether a launch of nCells cells runs four cells per wavefront (the
 // evaluator's shape, or the automatic choice); N: the kernels' size.
 bool team16_selected(const gfc_evaluator* ev, int nCells, int& N);
+// Row capacity N of the wavefront-per-cell kernel that can run this
+// evaluator's mechanism (0: none; nSpecie+1 <= 64, at most 3 species per
+// reaction side and 128 efficiency sets, gpuFoam's rate types only).
+int wave_kernel_rows(const gfc_evaluator* ev);
 } // namespace gfc

@@ -230,5 +230,32 @@ __device__ __forceinline__ double forward_rate(const Rxn& x, const TFun& f,
     return kInf * temp;
 }
 
+// One column's efficiencies of the first kSetsT sets (a uniform address,
+// read with vector loads so that the loads of the next column stay in
+// flight across the LDS traffic of this one).
+__device__ __forceinline__ void dense_row_load(const double* e, double (&out)[kSetsT]) {
+    int z = 0;  // a per-lane (zero) offset keeps these global vector loads
+    asm volatile("" : "+v"(z));
+    const double2* e2 = reinterpret_cast<const double2*>(e) + z;
+#pragma unroll
+    for (int q = 0; q < kSetsT / 2; ++q) {
+        const double2 v = e2[q];
+        out[2 * q] = v.x;
+        out[2 * q + 1] = v.y;
+    }
+}
+
+// sum_q w[q] e[q] as two interleaved partial sums (even and odd q)
+__device__ __forceinline__ double dense_dot(const double (&w)[kSetsT],
+                                            const double (&e)[kSetsT]) {
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int q = 0; q < kSetsT; q += 2) {
+        a0 += w[q] * e[q];
+        a1 += w[q + 1] * e[q + 1];
+    }
+    return a0 + a1;
+}
+
 } // namespace wave
 } // namespace gfc

@@ -26,448 +26,24 @@
 // n..N-1 are identity rows.
 #include <hip/hip_runtime.h>
 
-#include "team_common.hpp"
+#include "team16_common.hpp"
+#include "team_device.hpp"
 
 namespace gfc {
 
-namespace {
-
 using namespace wave;
-
-template <int N>
-struct TeamLDS {
-    static constexpr int P = (N % 2 == 0) ? N + 1 : N; // odd row pitch
-    double J[N * P];
-    double c[kLanes];
-    double dN[kLanes];
-    double Ha[kLanes];
-    double colA[kLanes];            // rhoM / W_j
-    double colB[kLanes];            // rhoM * v_j
-    double M[kTeamMaxSets];         // third-body concentration per set
-    double gL[kLanes], gH[kLanes];  // W_i Gstd_i with the low / high polynomial
-    double hL[kLanes], hH[kLanes];  // W_i Ha_i with the low / high polynomial
-    double Wd[kMaxSets * kLanes];   // dense Jacobian weights per set, specie
-};
-
-// Third-body concentrations of every efficiency set, M[s] = sum_i eff_si c_i
-// (gpuThirdBodyEfficiencies.H:21-27, summed in specie order): lane s sums
-// set s, so the sets are computed side by side.  L.c must hold c.
-template <int N>
-__device__ __forceinline__ void third_body(const DeviceMech& m, TeamLDS<N>& L, int lane) {
-    __syncthreads();
-    const int nS = m.nS;
-    for (int s = lane; s < m.nSets; s += kLanes) {
-        const double* e = m.effSets + s * kLanes;
-        double acc = 0.0;
-#pragma unroll 4
-        for (int i = 0; i < nS; ++i) acc += e[i] * L.c[i];
-        L.M[s] = acc;
-    }
-}
-
-// Specie Gibbs energies (and enthalpies) with both polynomial sets, times W:
-// a reaction's Y*Gstd of its composite thermo (gpuThermo.H:354-470,
-// makeGpuReactions.H:18-59) is the stoichiometric sum of these with the set
-// chosen by the composite's Tcommon.
-template <int N, bool ENTHALPY>
-__device__ __forceinline__ void specie_gibbs(const DeviceMech& m, TeamLDS<N>& L,
-                                             const TFun& f, int lane) {
-    if (lane < m.nS) {
-        const double W = m.W[lane];
-        const double* lo = m.low + 7 * lane;
-        const double* hi = m.high + 7 * lane;
-        L.gL[lane] = W * poly_Gstd(lo, f.T, f.logT);
-        L.gH[lane] = W * poly_Gstd(hi, f.T, f.logT);
-        if (ENTHALPY) {
-            L.hL[lane] = W * poly_Ha(lo, f.T);
-            L.hH[lane] = W * poly_Ha(hi, f.T);
-        }
-    }
-}
-
-// ---------------------------------------------------------- derivatives --
-// gpuODESystem::derivatives (gpuODESystem.H:164-196).  yl: component of
-// this lane; returns the dydx component of this lane.
-template <int N>
-__device__ __forceinline__ double team_derivatives(const DeviceMech& m, double p,
-                                                   double T, double yl,
-                                                   TeamLDS<N>& L, int lane) {
-    const int nS = m.nS;
-    double rhoPart = 0.0, cpPart = 0.0, Ha = 0.0, cu = 0.0;
-    if (lane < nS) {
-        const double yi = fmax(yl, 0.0);
-        const double rhoi = p / (m.R[lane] * T);
-        const double* a = coeffs_of(m.high + 7 * lane, m.low + 7 * lane, m.Tcommon[lane], T);
-        rhoPart = yi / rhoi;
-        cpPart = yi * poly_Cp(a, T);
-        Ha = poly_Ha(a, T);
-        cu = yi / m.W[lane];
-    }
-    const double rhoM = 1.0 / wave_sum(rhoPart);
-    const double CpM = wave_sum(cpPart);
-    const double cl = cu * rhoM;
-    L.c[lane] = cl;
-    L.dN[lane] = 0.0;
-    third_body<N>(m, L, lane);
-    // omega() evaluates the rates at T clipped to [Tlow, Thigh]
-    const TFun f = tfun(fmin(fmax(T, m.Tlow), m.Thigh));
-    specie_gibbs<N, false>(m, L, f, lane);
-    __syncthreads();
-#pragma unroll 1
-    for (int pass = 0; pass < m.nPass; ++pass) {
-        const int r = pass * kLanes + lane;
-        if (r < m.nR) {
-            const Rxn x = load_rxn(m, pass, lane);
-            double kArr, ddT, pw[kTeamWidth];
-            const double kf = forward_rate(x, f, L.M, kArr, ddT);
-            const double kr = x.irr ? 0.0 : kf * exp(-ln_kc(x, f, L, nullptr));
-            const double Cf = side_powers(x.li, x.lei, x.le, x.nl, L.c, pw);
-            const double Cr = side_powers(x.ri, x.rei, x.re, x.nr, L.c, pw);
-            const double w = kf * Cf - kr * Cr;
-#pragma unroll
-            for (int q = 0; q < kTeamWidth; ++q) {
-                if (q < x.nl) atomicAdd(&L.dN[x.li[q]], -(x.ls[q] * w));
-                if (q < x.nr) atomicAdd(&L.dN[x.ri[q]], x.rs[q] * w);
-            }
-        }
-    }
-    __syncthreads();
-    double d = 0.0;
-    if (lane < nS) d = L.dN[lane] * (m.W[lane] / rhoM);
-    const double dT = -wave_sum(d * Ha) / CpM;
-    if (lane == nS) d = dT;
-    return d;
-}
-
-// ------------------------------------------------------------- jacobian --
-// gpuODESystem::jacobian (gpuODESystem.H:198-281).  Leaves J (n x n, the
-// pressure row/column dropped) in L.J and returns this lane's component of
-// the derivative computed alongside (the reference's "dfdx").
-template <int N>
-__device__ __forceinline__ double team_jacobian(const DeviceMech& m, double p, double T,
-                                             double yl, TeamLDS<N>& L, int lane) {
-    constexpr int P = TeamLDS<N>::P;
-    const int nS = m.nS, n = nS + 1;
-    double rhoPart = 0, cpPart = 0, dcpPart = 0, avPart = 0, Ha = 0, Cp = 0, cu = 0;
-    if (lane < nS) {
-        const double yi = fmax(yl, 0.0);
-        const double rhoi = p / (m.R[lane] * T);
-        const double vi = 1.0 / rhoi;
-        const double* a = coeffs_of(m.high + 7 * lane, m.low + 7 * lane, m.Tcommon[lane], T);
-        Cp = poly_Cp(a, T);
-        Ha = poly_Ha(a, T);
-        rhoPart = yi / rhoi;
-        cpPart = yi * Cp;
-        dcpPart = yi * poly_dCpdT(a, T);
-        avPart = yi * vi * (1.0 / T);
-        cu = yi / m.W[lane];
-    }
-    const double rhoM = 1.0 / wave_sum(rhoPart);
-    const double CpM = wave_sum(cpPart);
-    const double dCpMdT = wave_sum(dcpPart);
-    const double alphavM = wave_sum(avPart) * rhoM;
-    const double cl = cu * rhoM;
-    L.c[lane] = cl;
-    L.dN[lane] = 0.0;
-    L.Ha[lane] = Ha;
-    if (lane < nS) {
-        L.colA[lane] = rhoM / m.W[lane];
-        L.colB[lane] = rhoM * (1.0 / (p / (m.R[lane] * T)));
-    }
-    third_body<N>(m, L, lane);
-    const TFun f = tfun(T);
-    specie_gibbs<N, true>(m, L, f, lane);
-    for (int e = lane; e < n * P; e += kLanes) L.J[e] = 0.0;
-    const int nChunks = m.nSets > 0 ? (m.nSets + kMaxSets - 1) / kMaxSets : 1;
-    const bool fusedDense = m.nSets <= kSetsT;
-    bool poisoned = false; // a fall-off rate with Cf - Cr/Kc not finite
-
-#pragma unroll 1
-    for (int chunk = 0; chunk < nChunks; ++chunk) {
-        for (int e = lane; e < kMaxSets * kLanes; e += kLanes) L.Wd[e] = 0.0;
-        __syncthreads();
-        const int s0 = chunk * kMaxSets;
-#pragma unroll 1
-        for (int pass = 0; pass < m.nPass; ++pass) {
-            const int r = pass * kLanes + lane;
-            if (r >= m.nR) continue;
-            const Rxn x = load_rxn(m, pass, lane);
-            const bool denseHere =
-                x.type == GFC_RATE_THIRD_BODY && x.set >= s0 && x.set < s0 + kMaxSets;
-            // sparse part in the first chunk; dense part with its set's chunk
-            if (chunk > 0 && !denseHere) continue;
-            double kArr, ddT, pwl[kTeamWidth], pwr[kTeamWidth];
-            const double kf = forward_rate(x, f, L.M, kArr, ddT);
-            const double Cf = side_powers(x.li, x.lei, x.le, x.nl, L.c, pwl);
-            double dwdT = Cf * ddT;
-            double w = Cf * kf;
-            double Cr = 0.0, Kc = 1.0, kr = 0.0;
-            if (!x.irr) {
-                double dKc;
-                const double lnKc = ln_kc(x, f, L, &dKc);
-                Kc = exp(lnKc);
-                const double rKc = exp(-lnKc);
-                kr = kf * rKc;
-                Cr = side_powers(x.ri, x.rei, x.re, x.nr, L.c, pwr);
-                const double dkrdT = ddT * rKc - kr * dKc;
-                dwdT -= Cr * dkrdT;
-                w -= Cr * kr;
-            }
-            const double temp = (Cf - Cr / Kc);
-            if (denseHere) {
-                // ddc_j = eff_j k (Cf - Cr/Kc): row weights of the set
-                const double g = kArr * temp;
-                double* Wd = &L.Wd[(x.set - s0) * kLanes];
-#pragma unroll
-                for (int i = 0; i < kTeamWidth; ++i) {
-                    if (i < x.nl) atomicAdd(&Wd[x.li[i]], -(x.ls[i] * g));
-                    if (i < x.nr) atomicAdd(&Wd[x.ri[i]], x.rs[i] * g);
-                }
-            }
-            if (chunk > 0) continue;
-            if (x.type > GFC_RATE_THIRD_BODY && !isfinite(temp)) poisoned = true;
-            // dN/dt (the reference's dfdx) and dCdT (gpuReaction.H:595-612)
-#pragma unroll
-            for (int i = 0; i < kTeamWidth; ++i) {
-                if (i < x.nl) {
-                    atomicAdd(&L.dN[x.li[i]], -(x.ls[i] * w));
-                    atomicAdd(&L.J[x.li[i] * P + nS], -(x.ls[i] * dwdT));
-                }
-                if (i < x.nr) {
-                    atomicAdd(&L.dN[x.ri[i]], x.rs[i] * w);
-                    atomicAdd(&L.J[x.ri[i] * P + nS], x.rs[i] * dwdT);
-                }
-            }
-            // jac_dCfdcj_contribution (gpuReaction.H:547-569)
-#pragma unroll
-            for (int j = 0; j < kTeamWidth; ++j) {
-                if (j >= x.nl) break;
-                const double dCf = side_dpow(x.li, x.lei, x.le, x.nl, j, L.c, pwl) * kf;
-                const int sj = x.li[j];
-#pragma unroll
-                for (int i = 0; i < kTeamWidth; ++i) {
-                    if (i < x.nl) atomicAdd(&L.J[x.li[i] * P + sj], -(x.ls[i] * dCf));
-                    if (i < x.nr) atomicAdd(&L.J[x.ri[i] * P + sj], x.rs[i] * dCf);
-                }
-            }
-            // jac_dCrdcj_contribution (:571-593)
-            if (!x.irr) {
-#pragma unroll
-                for (int j = 0; j < kTeamWidth; ++j) {
-                    if (j >= x.nr) break;
-                    const double dCr = side_dpow(x.ri, x.rei, x.re, x.nr, j, L.c, pwr) * kr;
-                    const int sj = x.ri[j];
-#pragma unroll
-                    for (int i = 0; i < kTeamWidth; ++i) {
-                        if (i < x.nl) atomicAdd(&L.J[x.li[i] * P + sj], x.ls[i] * dCr);
-                        if (i < x.nr) atomicAdd(&L.J[x.ri[i] * P + sj], -(x.rs[i] * dCr));
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        // dense third-body part, J(i, j) += sum_s Wd[s][i] eff_s[j]: with at
-        // most kSetsT sets it is added row-parallel inside the ddY_dcdY
-        // transform below; otherwise column-parallel here, one chunk of
-        // kMaxSets sets at a time
-        if (!fusedDense && lane < nS) {
-            const int ns = min(kMaxSets, m.nSets - s0);
-            double e[kMaxSets];
-#pragma unroll
-            for (int s = 0; s < kMaxSets; ++s)
-                e[s] = (s < ns) ? m.effSets[(s0 + s) * kLanes + lane] : 0.0;
-#pragma unroll 2
-            for (int i = 0; i < nS; ++i) {
-                double acc = 0.0;
-#pragma unroll
-                for (int s = 0; s < kMaxSets; ++s) acc += L.Wd[s * kLanes + i] * e[s];
-                L.J[i * P + lane] += acc;
-            }
-        }
-        __syncthreads();
-    }
-    // a fall-off reaction with a non-finite Cf - Cr/Kc makes the reference's
-    // dense (zero) concentration derivative NaN for its species rows
-    if (__any(poisoned)) {
-        const double qnan = __longlong_as_double(0x7ff8000000000000LL);
-#pragma unroll 1
-        for (int pass = 0; pass < m.nPass; ++pass) {
-            const int r = pass * kLanes + lane;
-            if (r < m.nR) {
-                const Rxn x = load_rxn(m, pass, lane);
-                if (x.type > GFC_RATE_THIRD_BODY) {
-                    double pw[kTeamWidth];
-                    const double Cf = side_powers(x.li, x.lei, x.le, x.nl, L.c, pw);
-                    const double Cr = side_powers(x.ri, x.rei, x.re, x.nr, L.c, pw);
-                    const double temp = Cf - Cr / exp(ln_kc(x, f, L, nullptr));
-                    if (!isfinite(temp)) {
-                        for (int i = 0; i < x.nl; ++i)
-                            for (int j = 0; j < nS; ++j) L.J[x.li[i] * P + j] = qnan;
-                        for (int i = 0; i < x.nr; ++i)
-                            for (int j = 0; j < nS; ++j) L.J[x.ri[i] * P + j] = qnan;
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
-    // derivative: dY/dt and dT/dt
-    double d = 0.0;
-    if (lane < nS) d = L.dN[lane] * (m.W[lane] / rhoM);
-    const double dfdxT = -wave_sum(d * Ha) / CpM;
-    // ddY_dcdY (gpuODESystem.H:65-103): lane a transforms row a
-    if (lane < nS) {
-        double* row = &L.J[lane * P];
-        const double WiByrhoM = m.W[lane] / rhoM;
-        double ddNdT = row[nS];
-        if (fusedDense) {
-            // this row's weight of every set; the sets' efficiencies of
-            // column j are uniform (scalar loads of the transposed table)
-            double w[kSetsT];
-#pragma unroll
-            for (int q = 0; q < kSetsT; ++q) w[q] = L.Wd[q * kLanes + lane];
-#pragma unroll 2
-            for (int j = 0; j < nS; ++j) {
-                const double* e = m.effSetsT + j * kSetsT;
-                double acc = 0.0;
-#pragma unroll
-                for (int q = 0; q < kSetsT; ++q) acc += w[q] * e[q];
-                const double tj = row[j] + acc;
-                ddNdT -= tj * L.c[j] * alphavM;
-                row[j] = WiByrhoM * tj * L.colA[j] + L.colB[j] * d;
-            }
-        } else {
-#pragma unroll 4
-            for (int j = 0; j < nS; ++j) {
-                const double tj = row[j];
-                ddNdT -= tj * L.c[j] * alphavM;
-                row[j] = WiByrhoM * tj * L.colA[j] + L.colB[j] * d;
-            }
-        }
-        row[nS] = WiByrhoM * ddNdT + alphavM * d;
-    }
-    __syncthreads();
-    // ddT_dtdY (:108-126): lane i sums column i
-    if (lane < nS) {
-        double v = 0.0;
-#pragma unroll 4
-        for (int j = 0; j < nS; ++j) v -= L.J[j * P + lane] * L.Ha[j];
-        v -= Cp * dfdxT;
-        L.J[nS * P + lane] = v / CpM;
-    }
-    // ddT_dtdT (:129-146)
-    const double term = (lane < nS) ? d * Cp + L.J[lane * P + nS] * Ha : 0.0;
-    const double s = wave_sum(term);
-    if (lane == 0) L.J[nS * P + nS] = (-s - dfdxT * dCpMdT) / CpM;
-    __syncthreads();
-    if (lane == nS) d = dfdxT;
-    return d;
-}
-
-// --------------------------------------------------------------- LU ----
-// Row-per-lane factorisation.  On return `ps` is the step at which this
-// lane's row was the pivot, `rd` the reciprocal of its pivot and lane j's
-// `plv` the pivot lane of step j.
-template <int N>
-__device__ __forceinline__ void team_lu(double (&row)[N], int n, int lane, int& ps,
-                                        double& rd, int& plv) {
-    double amax = 0.0;
-#pragma unroll
-    for (int k = 0; k < N; ++k) amax = fmax(amax, fabs(row[k]));
-    const double vv = 1.0 / amax;
-    int pos = lane;
-    bool done = lane >= N;
-    ps = lane;
-    rd = 1.0;
-    plv = lane;
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        // candidates: rows not yet pivots (padding rows only once the real
-        // columns are done); NaN keys never win (the reference's >=)
-        double key = (done || (lane >= n && j < n)) ? -1.0 : vv * fabs(row[j]);
-        if (!(key >= 0.0)) key = -1.0;
-        const double kmax = wave_max(key);
-        // one candidate (the rule case) needs no tie-break reduction; ties go
-        // to the largest row position
-        const unsigned long long cand = __ballot(key == kmax && !done);
-        int pv;
-        if (__builtin_popcountll(cand) == 1) {
-            pv = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(cand));
-        } else {
-            const int Pt = wave_max_i((key == kmax && !done) ? pos : -1);
-            pv = __builtin_amdgcn_readfirstlane(
-                (int)__builtin_ctzll(__ballot(pos == Pt && !done)));
-        }
-        const int P = __builtin_amdgcn_readlane(pos, pv);
-        if (lane == j) plv = pv;
-        if (pos == j) pos = P;
-        if (lane == pv) {
-            pos = j;
-            done = true;
-            ps = j;
-        }
-        {
-            // settle this step's per-lane bookkeeping here: left as select
-            // chains, every step's lane masks stay live to the end of the
-            // factorisation (SGPRs spilled into VGPR lanes)
-            int dn = done ? 1 : 0;
-            asm volatile("" : "+v"(ps), "+v"(rd), "+v"(plv), "+v"(pos), "+v"(dn));
-            done = dn != 0;
-        }
-        double dj = readlane(row[j], pv);
-        if (dj == 0.0) {
-            dj = kSmall;
-            if (lane == pv) row[j] = kSmall;
-        }
-        const double rDiag = 1.0 / dj;
-        if (lane == pv) rd = rDiag;
-        const double l = done ? 0.0 : row[j] * rDiag;
-        if (!done) row[j] = l;
-        // update in groups of 8 columns; the scheduling barrier keeps the
-        // broadcast values of one group in flight at a time
-#pragma unroll
-        for (int k = j + 1; k < N; ++k) {
-            row[k] = fma(-l, readlane(row[k], pv), row[k]);
-            if ((k - j) % 8 == 0) __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-}
-
-// Solves (LU) x = b for b held species-parallel; returns x species-parallel.
-template <int N>
-__device__ __forceinline__ double team_solve(const double (&row)[N], int lane, int ps,
-                                             double rd, int plv, double b) {
-    // keep the per-step pivot lanes from being hoisted (and held in scalar
-    // registers) across the calls of one attempt
-    asm volatile("" : "+v"(plv));
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const int pv = __builtin_amdgcn_readlane(plv, j);
-        const double yj = readlane(b, pv);
-        if (ps > j) b = fma(-row[j], yj, b);
-    }
-    double x = 0.0;
-#pragma unroll
-    for (int j = N - 1; j >= 0; --j) {
-        const int pv = __builtin_amdgcn_readlane(plv, j);
-        const double xj = readlane(b * rd, pv);
-        if (ps < j) b = fma(-row[j], xj, b);
-        if (lane == j) x = xj;
-    }
-    return x;
-}
-
-} // namespace
 
 // ---------------------------------------------------------------- kernel --
 
-// Small mechanisms (N <= 16) fit in 256 registers: two wavefronts per SIMD
-// hide each other's latency.  Larger ones need the AGPRs as well and run
-// one wavefront per SIMD.
-template <int N>
-__global__ void __launch_bounds__(64, (N <= 16) ? 2 : 1)
-team_solve_kernel(SolveArgs A, int* queue) {
-    __shared__ TeamLDS<N> L;
+// The persistent (queue) or drop-in (queue == nullptr: cell = block index)
+// solve of one wavefront.  With REC it also records the accepted steps for
+// the discrete adjoint (the per-cell arrays of t16::T16Record, wave-blocked):
+// the initial state, every accepted step's size, every K-th accepted step's
+// initial state, the final state before the non-negativity clamp and the
+// number of accepted steps.
+template <int N, bool REC>
+__device__ __forceinline__ void team_run(const SolveArgs& A, int* queue, const t16::T16Record& R,
+                                         TeamLDS<N>& L) {
     const int lane = threadIdx.x;
     const DeviceMech& m = A.m;
     const int nS = m.nS, n = nS + 1;
@@ -510,6 +86,12 @@ team_solve_kernel(SolveArgs A, int* queue) {
         const double p = A.Y0[(long)(nS + 1) * ld + cell];
         double dtChem = A.dtChem[cell];
         int status = 0, attempts = 0, accepted = 0;
+        int count = 0; // recorded accepted steps
+        if constexpr (REC) {
+            double* y0c = t16::rec_slot(R.y0c, cell, n + 1);
+            if (lane < n) y0c[lane * t16::kRecStride] = y;
+            if (lane == 0) y0c[(long)n * t16::kRecStride] = p;
+        }
         if (readlane(y, nS) > A.Treact) {
             double timeLeft = A.deltaT;
             while (timeLeft > kSmall) {
@@ -601,6 +183,15 @@ team_solve_kernel(SolveArgs A, int* queue) {
                     }
                     if (status & GFC_CELL_STEP_UNDERFLOW) break;
                     ++accepted;
+                    if constexpr (REC) {
+                        // the state at the start of the step and its size
+                        if (lane == 0)
+                            t16::rec_slot(R.hs, cell, R.maxSteps)[(long)count * t16::kRecStride] = dx;
+                        if (count % R.K == 0 && lane < n)
+                            t16::rec_slot(R.ck, cell, R.nck * n)[((long)(count / R.K) * n + lane) *
+                                                                 t16::kRecStride] = y;
+                        ++count;
+                    }
                     x += dx;
                     y = yNew;
                     if (err > A.incThreshold) {
@@ -618,6 +209,9 @@ team_solve_kernel(SolveArgs A, int* queue) {
                 }
                 if (!reached && !(status & GFC_CELL_STEP_UNDERFLOW))
                     status |= GFC_CELL_MAX_STEPS;
+                if constexpr (REC) {
+                    if (lane < n) t16::rec_slot(R.yEnd, cell, n)[lane * t16::kRecStride] = y;
+                }
                 if (lane < nS) y = fmax(0.0, y);
                 timeLeft -= xEnd;
                 if (status & GFC_CELL_STEP_UNDERFLOW) break;
@@ -628,6 +222,9 @@ team_solve_kernel(SolveArgs A, int* queue) {
         }
         if (A.RR && lane < nS)
             A.RR[lane * ld + cell] = A.rho[cell] * (y - y0cell) / A.deltaT;
+        if constexpr (REC) {
+            if (lane == 0) R.nAcc[cell] = count;
+        }
         if (lane == 0 && A.dtMinBits) atomic_min_positive(A.dtMinBits, dtChem);
         const int slot = cell - cBase;
         const double yOut = (lane < n) ? y : p;
@@ -659,6 +256,24 @@ team_solve_kernel(SolveArgs A, int* queue) {
         if (!queue) break;
         __syncthreads();
     }
+}
+
+// Small mechanisms (N <= 16) fit in 256 registers: two wavefronts per SIMD
+// hide each other's latency.  Larger ones need the AGPRs as well and run
+// one wavefront per SIMD.
+template <int N>
+__global__ void __launch_bounds__(64, (N <= 16) ? 2 : 1)
+team_solve_kernel(SolveArgs A, int* queue) {
+    __shared__ TeamLDS<N> L;
+    team_run<N, false>(A, queue, t16::T16Record{}, L);
+}
+
+// The same solve recording its accepted steps for the discrete adjoint.
+template <int N>
+__global__ void __launch_bounds__(64, (N <= 16) ? 2 : 1)
+team_record_kernel(SolveArgs A, int* queue, t16::T16Record R) {
+    __shared__ TeamLDS<N> L;
+    team_run<N, true>(A, queue, R, L);
 }
 
 // Device-level evaluations with this kernel's own physics and LU (one cell
@@ -711,5 +326,9 @@ template __global__ void team_solve_kernel<11>(SolveArgs, int*);
 template __global__ void team_solve_kernel<32>(SolveArgs, int*);
 template __global__ void team_solve_kernel<54>(SolveArgs, int*);
 template __global__ void team_solve_kernel<64>(SolveArgs, int*);
+template __global__ void team_record_kernel<11>(SolveArgs, int*, t16::T16Record);
+template __global__ void team_record_kernel<32>(SolveArgs, int*, t16::T16Record);
+template __global__ void team_record_kernel<54>(SolveArgs, int*, t16::T16Record);
+template __global__ void team_record_kernel<64>(SolveArgs, int*, t16::T16Record);
 
 } // namespace gfc

@@ -1,0 +1,574 @@
+// Discrete adjoint with one wavefront per cell (nSpecie+1 <= 64; no
+// counterpart in gpuFoam).
+//
+// The reverse sweep of adjoint.hip with the wavefront kernel's mapping: a
+// wavefront takes cells from a queue and walks a cell's recorded accepted
+// steps backwards, checkpoint segment by segment (a segment is
+// re-integrated from its checkpoint first).  Per step it recomputes the
+// Jacobian, LU and stages with the wavefront solve's step code
+// (team_device.hpp), then applies the reverse of the step as
+// adjoint_math.hpp's step_reverse does:
+//   * lane i holds component i of every vector;
+//   * the step's Jacobian (transformed and, in raw, its species rows before
+//     the transform), its LU factors (rows in pivot order) and the adjoint
+//     of the Jacobian (Jb) live in LDS;
+//   * transposed solves read the LU from LDS, one position per broadcast;
+//   * reaction sweeps run 64 reactions per pass, one per lane, with the
+//     one-lane form's per-reaction reverse (reaction_reverse_w) and LDS
+//     atomics for the concentration adjoints;
+//   * wavefront sums are DPP trees, so sums differ from the one-lane form's
+//     sequential order in the last bits.
+// The sweeps use the wavefront solve's tables, reactions grouped by rate
+// type; rate-parameter adjoints go to each reaction's reference index
+// (rxnRef).  The LDS of a cell (four n x n tiles) holds one wavefront per
+// compute unit for GRI-Mech's n = 54.
+#include <hip/hip_runtime.h>
+
+#include "adjoint_math.hpp"
+#include "team16_common.hpp"
+#include "team_device.hpp"
+
+namespace gfc {
+
+namespace adj {
+// LDS accumulation shared by the lanes of a wavefront (reaction_reverse_w)
+struct WaveAcc {
+    double* p;
+};
+__device__ __forceinline__ void acc_add(WaveAcc v, int i, double x) { atomicAdd(&v.p[i], x); }
+} // namespace adj
+
+namespace {
+
+using t16::kRecStride;
+using t16::rec_slot;
+using t16::T16Record;
+using t16::wave_sync;
+
+template <int N>
+struct WaveAdjLDS {
+    static constexpr int P = TeamLDS<N>::P;
+    TeamLDS<N> f;       // the solve's arrays; f.J holds the step's Jacobian
+    double raw[N * P];  // species rows before the transform
+    double LU[N * P];   // LU factors, row j = pivot position j
+    double Jb[N * P];   // adjoint of the Jacobian
+    double cb[kLanes];  // concentration adjoints
+    double dNb[kLanes]; // adjoints of dN_i/dt
+    double vj[kLanes];  // 1/rho_i of each specie
+    double v[kLanes];   // a vector being broadcast
+};
+
+__device__ __forceinline__ double bcast_from(double v, int src) {
+    const int lo = __shfl(__double2loint(v), src);
+    const int hi = __shfl(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+
+// One step's recomputed quantities, this lane's components.
+struct WStages {
+    double F, dydx0, k1, k2, k3, k4, f2, f3;
+    int ps; // pivot position of this lane's row
+};
+
+// The accepted attempt of a step of size dx from y, as the solve computes it
+// (team_kernel.hip): returns the new state.  With KEEP, also leaves the
+// pre-transform rows in L.raw and the LU in L.LU.
+template <int N, bool KEEP>
+__device__ __forceinline__ double w_step(const SolveArgs& A, WaveAdjLDS<N>& L, double p,
+                                         double y, double dx, int lane, WStages& S) {
+    constexpr int P = WaveAdjLDS<N>::P;
+    const DeviceMech& m = A.m;
+    const int nS = m.nS, n = nS + 1;
+    const RosTableau& tb = A.tab;
+    const double T = readlane(y, nS);
+    const double dfdx = team_jacobian<N, KEEP>(m, p, T, y, L.f, lane, L.raw);
+    const double dydx0 = (T >= m.Tlow && T <= m.Thigh)
+                             ? dfdx
+                             : team_derivatives<N>(m, p, T, y, L.f, lane);
+    // -J + I/(gamma dx) as the solve builds it
+    double row[N];
+    const double tdiag = 1.0 / (tb.gamma * dx);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double v = 0.0;
+        if (k < n && lane < n) v = -L.f.J[lane * P + k];
+        if (k == lane) v += (lane < n) ? tdiag : 1.0;
+        row[k] = v;
+    }
+    int ps, plv;
+    double rd;
+    team_lu<N>(row, n, lane, ps, rd, plv);
+    if constexpr (KEEP) {
+        // (a NaN Jacobian can leave a row at a padding position >= n)
+        if (lane < n && ps < n) {
+#pragma unroll
+            for (int k = 0; k < N; ++k)
+                if (k < n) L.LU[ps * P + k] = row[k];
+        }
+    }
+    double k1 = 0, k2 = 0, k3 = 0, k4 = 0, dydx = 0, f2 = 0, f3 = 0;
+#pragma unroll 1
+    for (int st = 0; st < tb.stages; ++st) {
+        if (st == 1 || (st == 2 && tb.stages == 4)) {
+            const double yt = (st == 1) ? y + tb.a21 * k1 : y + tb.a31 * k1 + tb.a32 * k2;
+            dydx = team_derivatives<N>(m, p, readlane(yt, nS), yt, L.f, lane);
+            if (st == 1) f2 = dydx;
+            else f3 = dydx;
+        }
+        double rhs;
+        if (st == 0) rhs = dydx0 + dx * tb.d1 * dfdx;
+        else if (st == 1) rhs = dydx + dx * tb.d2 * dfdx + tb.c21 * k1 / dx;
+        else if (st == 2) rhs = dydx + dx * tb.d3 * dfdx + (tb.c31 * k1 + tb.c32 * k2) / dx;
+        else rhs = dydx + dx * tb.d4 * dfdx + (tb.c41 * k1 + tb.c42 * k2 + tb.c43 * k3) / dx;
+        const double kx = team_solve<N>(row, lane, ps, rd, plv, rhs);
+        if (st == 0) k1 = kx;
+        else if (st == 1) k2 = kx;
+        else if (st == 2) k3 = kx;
+        else k4 = kx;
+    }
+    S.F = dfdx;
+    S.dydx0 = dydx0;
+    S.k1 = k1;
+    S.k2 = k2;
+    S.k3 = k3;
+    S.k4 = k4;
+    S.f2 = f2;
+    S.f3 = f3;
+    S.ps = ps;
+    wave_sync();
+    if (tb.stages == 2) return y + tb.b1 * k1 + tb.b2 * k2;
+    if (tb.stages == 3) return y + tb.b1 * k1 + tb.b2 * k2 + tb.b3 * k3;
+    return y + tb.b1 * k1 + tb.b2 * k2 + tb.b3 * k3 + tb.b4 * k4;
+}
+
+// Solves A^T x = r for the step's A = P^T L U (adjoint_math.hpp lu_solve_T):
+// lane i holds r_i and receives x_i.
+template <int N>
+__device__ __forceinline__ double w_solveT(const WaveAdjLDS<N>& L, int n, int ps, double r,
+                                           int lane) {
+    constexpr int P = WaveAdjLDS<N>::P;
+    // U^T w = r
+    double w = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const double wj = readlane(r, j) / L.LU[j * P + j];
+        if (lane == j) w = wj;
+        if (lane > j && lane < n) r -= L.LU[j * P + lane] * wj;
+    }
+    // L^T v = w (unit diagonal)
+    double v = 0.0;
+    for (int k = n - 1; k >= 0; --k) {
+        const double vk = readlane(w, k);
+        if (lane == k) v = vk;
+        if (lane < k) w -= L.LU[k * P + lane] * vk;
+    }
+    // x = P^T v: row i sits at position ps_i
+    return bcast_from(v, ps);
+}
+
+// Jb(i, j) += r_i k_j
+template <int N>
+__device__ __forceinline__ void w_add_outer(WaveAdjLDS<N>& L, int n, double r, double k,
+                                            int lane) {
+    constexpr int P = WaveAdjLDS<N>::P;
+    L.v[lane] = k;
+    wave_sync();
+    if (lane < n && r != 0.0)
+        for (int j = 0; j < n; ++j) L.Jb[lane * P + j] += r * L.v[j];
+    wave_sync();
+}
+
+// Mixture quantities of the cell at y (adjoint_math.hpp mixture): wavefront
+// scalars, and this lane's specie terms.
+struct WMix {
+    double S1, rho, CpM, dCpMdT, avRaw, av;
+    double yc, vi, Cp, dCp, Ha, c;
+};
+__device__ __forceinline__ WMix w_mixture(const DeviceMech& m, double p, double y, double T,
+                                          int lane) {
+    WMix x{};
+    double S1p = 0, CpMp = 0, dCpp = 0, avp = 0, cu = 0;
+    if (lane < m.nS) {
+        x.yc = fmax(y, 0.0);
+        const double rhoi = p / (m.R[lane] * T);
+        x.vi = 1.0 / rhoi;
+        const double* a = coeffs_of(m.high + 7 * lane, m.low + 7 * lane, m.Tcommon[lane], T);
+        x.Cp = poly_Cp(a, T);
+        x.dCp = poly_dCpdT(a, T);
+        x.Ha = poly_Ha(a, T);
+        S1p = x.yc / rhoi;
+        CpMp = x.yc * x.Cp;
+        dCpp = x.yc * x.dCp;
+        avp = x.yc * x.vi * (1.0 / T);
+        cu = x.yc / m.W[lane];
+    }
+    x.S1 = wave_sum(S1p);
+    x.rho = 1.0 / x.S1;
+    x.CpM = wave_sum(CpMp);
+    x.dCpMdT = wave_sum(dCpp);
+    x.avRaw = wave_sum(avp);
+    x.av = x.avRaw * x.rho;
+    x.c = cu * x.rho;
+    return x;
+}
+
+// Reverse of w_mixture (adjoint_math.hpp mixture_reverse): adds to this
+// lane's ybar; lane nS receives the temperature part plus TbExtra.
+__device__ __forceinline__ void w_mixture_reverse(const DeviceMech& m, double p, double y,
+                                                  double T, const WMix& x, double cb,
+                                                  double Cpb, double Hab, double vb, bool full,
+                                                  adj::MixBar mb, double TbExtra,
+                                                  double& ybar, int lane) {
+    const int nS = m.nS;
+    const double rho = x.rho;
+    const double arb = mb.alphav * rho;
+    double rhoPart = 0.0;
+    if (lane < nS) rhoPart = cb * (x.yc / m.W[lane]);
+    const double rhob = mb.rho + mb.alphav * x.avRaw + wave_sum(rhoPart);
+    const double S1b = -rhob * rho * rho;
+    double TbPart = 0.0, ycb = 0.0;
+    if (lane < nS) {
+        ycb = cb * rho / m.W[lane] + S1b * x.vi + arb * x.vi * (1.0 / T) + mb.CpM * x.Cp;
+        TbPart = Hab * x.Cp;
+        if (full) {
+            const double* a = coeffs_of(m.high + 7 * lane, m.low + 7 * lane, m.Tcommon[lane], T);
+            ycb += mb.dCpMdT * x.dCp;
+            TbPart += mb.dCpMdT * x.yc * adj::poly_d2CpdT2(a, T) + Cpb * x.dCp + vb * x.vi / T;
+        }
+    }
+    const double Tb = S1b * x.S1 / T + mb.CpM * x.dCpMdT + wave_sum(TbPart);
+    if (lane < nS && y >= 0.0) ybar += ycb; // right derivative of max(y, 0) at 0
+    if (lane == nS) ybar = (ybar + TbExtra) + Tb;
+}
+
+// ybar += d<fbar, f(y)>/dy, th += d<fbar, f(y)>/dtheta (adjoint_math.hpp
+// vjp_f); fy = f(y).
+template <int N>
+__device__ __forceinline__ void w_vjp_f(const DeviceMech& mr, WaveAdjLDS<N>& L, double p,
+                                        double y, double fbar, double fy, double& ybar,
+                                        const adj::ThView& th, int lane) {
+    const int nS = mr.nS;
+    const double T = readlane(y, nS);
+    const WMix x = w_mixture(mr, p, y, T, lane);
+    const double g = readlane(fbar, nS);
+    const double fT = readlane(fy, nS);
+    adj::MixBar mb;
+    mb.CpM = -g * fT / x.CpM;
+    double Hab = 0.0, rhoPart = 0.0;
+    if (lane < nS) {
+        Hab = -g * fy / x.CpM;
+        const double fbi = fbar - g * x.Ha / x.CpM;
+        L.dNb[lane] = fbi * mr.W[lane] / x.rho;
+        rhoPart = -fbi * fy / x.rho;
+    }
+    mb.rho = wave_sum(rhoPart);
+    L.cb[lane] = 0.0;
+    L.f.c[lane] = (lane < nS) ? x.c : 0.0;
+    wave_sync();
+    double Tb = 0.0;
+    const int nPass = (mr.nR + kLanes - 1) / kLanes;
+#pragma unroll 1
+    for (int pass = 0; pass < nPass; ++pass) {
+        const int r = pass * kLanes + lane;
+        if (r >= mr.nR) continue;
+        const long o = (long)r * kWidth;
+        adj::RxnBarT<kTeamWidth> in;
+        for (int q = 0; q < kTeamWidth && q < mr.nl[r]; ++q) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
+        for (int q = 0; q < kTeamWidth && q < mr.nr[r]; ++q) in.w += mr.rs[o + q] * L.dNb[mr.ri[o + q]];
+        if (in.w == 0.0) continue;
+        double t6[adj::kParams] = {0, 0, 0, 0, 0, 0};
+        adj::reaction_reverse_w<kTeamWidth>(mr, r, T, true, lane::SV{L.f.c, 1}, in,
+                                            adj::WaveAcc{L.cb}, Tb, t6);
+        th.add(mr.rxnRef[r], t6);
+    }
+    Tb = wave_sum(Tb);
+    wave_sync();
+    const double cb = (lane < nS) ? L.cb[lane] : 0.0;
+    w_mixture_reverse(mr, p, y, T, x, cb, 0.0, Hab, 0.0, false, mb, Tb, ybar, lane);
+    wave_sync();
+}
+
+// ybar += d/dy and th += d/dtheta of <Jb, J(y)> + <Fb, F(y)> for the fast
+// Jacobian J = L.f.J (pre-transform rows in L.raw) and F at y
+// (adjoint_math.hpp jact).  Jb is consumed.
+template <int N>
+__device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, double p,
+                                       double y, double F, double Fb, double& ybar,
+                                       const adj::ThView& th, int lane) {
+    constexpr int P = WaveAdjLDS<N>::P;
+    const int nS = mr.nS, iT = nS;
+    const double T = readlane(y, nS);
+    const WMix x = w_mixture(mr, p, y, T, lane);
+    const double CpM = x.CpM, rho = x.rho, av = x.av;
+    const double FT = readlane(F, nS);
+    const bool sp = lane < nS;
+    L.f.c[lane] = sp ? x.c : 0.0;
+    L.f.Ha[lane] = sp ? x.Ha : 0.0;
+    L.vj[lane] = sp ? x.vi : 0.0;
+    L.v[lane] = F;
+    double Cpb = 0.0, Hab = 0.0, vb = 0.0, cbT = 0.0;
+    adj::MixBar mb;
+    // ddT_dtdT (gpuODESystem.H:129-146)
+    {
+        const double g = L.Jb[iT * P + iT];
+        if (sp) {
+            Fb += -g * x.Cp / CpM;
+            Cpb += -g * F / CpM;
+            L.Jb[lane * P + iT] += -g * x.Ha / CpM;
+            Hab += -g * L.f.J[lane * P + iT] / CpM;
+        }
+        if (lane == iT) Fb += -g * x.dCpMdT / CpM;
+        mb.dCpMdT += -g * FT / CpM;
+        mb.CpM += -g * L.f.J[iT * P + iT] / CpM;
+    }
+    wave_sync();
+    // ddT_dtdY (:108-126): lane j walks the columns i of its row
+    {
+        if (sp) {
+            for (int i = 0; i < nS; ++i) {
+                const double g = L.Jb[iT * P + i];
+                if (g == 0.0) continue;
+                L.Jb[lane * P + i] += -g * x.Ha / CpM;
+                Hab += -g * L.f.J[lane * P + i] / CpM;
+            }
+        }
+        const double gi = sp ? L.Jb[iT * P + lane] : 0.0;
+        double fbT = 0.0, cpm = 0.0;
+        if (sp && gi != 0.0) {
+            Cpb += -gi * FT / CpM;
+            fbT = -gi * x.Cp / CpM;
+            cpm = -gi * L.f.J[iT * P + lane] / CpM;
+        }
+        fbT = wave_sum(fbT);
+        mb.CpM += wave_sum(cpm);
+        if (lane == iT) Fb += fbT;
+    }
+    wave_sync();
+    // ddY_dcdY (:65-103).  Column sums first, from the entries before the
+    // row transforms: vb_j (species columns) and the temperature column's
+    // concentration adjoints.
+    if (sp) {
+        for (int i = 0; i < nS; ++i) {
+            vb += L.Jb[i * P + lane] * rho * L.v[i];
+            const double g = L.Jb[i * P + iT];
+            cbT += -g * (mr.W[i] / rho) * L.raw[i * P + lane] * av;
+        }
+    }
+    wave_sync();
+    double rhoPart = 0.0, avPart = 0.0;
+    if (sp) {
+        const double WiR = mr.W[lane] / rho;
+        double* row = &L.Jb[lane * P];
+        for (int j = 0; j < nS; ++j) {
+            const double g = row[j];
+            rhoPart += g * L.vj[j] * F;
+            Fb += g * rho * L.vj[j];
+            row[j] = g * WiR * (rho / mr.W[j]);
+        }
+        // temperature column
+        const double g = row[iT];
+        double sumc = 0.0;
+        for (int j = 0; j < nS; ++j) sumc += L.raw[lane * P + j] * L.f.c[j];
+        const double X = L.raw[lane * P + iT] - sumc * av;
+        row[iT] = g * WiR;
+        for (int j = 0; j < nS; ++j) row[j] += -g * WiR * L.f.c[j] * av;
+        avPart = g * (F - WiR * sumc);
+        rhoPart += -g * mr.W[lane] / (rho * rho) * X;
+        Fb += g * av;
+    }
+    mb.rho += wave_sum(rhoPart);
+    mb.alphav += wave_sum(avPart);
+    // dfdx: F_T = -sum F_i Ha_i / CpM, F_i = dN_i W_i / rho
+    {
+        const double g = readlane(Fb, iT);
+        if (sp) {
+            Fb += -g * x.Ha / CpM;
+            Hab += -g * F / CpM;
+        }
+        mb.CpM += -g * FT / CpM;
+        double part = 0.0;
+        if (sp) {
+            L.dNb[lane] = Fb * mr.W[lane] / rho;
+            part = -Fb * F / rho;
+        }
+        mb.rho += wave_sum(part);
+    }
+    L.cb[lane] = sp ? cbT : 0.0;
+    wave_sync();
+    // reactions (gpuReaction.H:547-702): adjoints of omega, dOmega/dT,
+    // kf dCf/dc_j, kr dCr/dc_j and D from the raw-entry adjoints
+    double Tb = 0.0;
+    const int nPass = (mr.nR + kLanes - 1) / kLanes;
+#pragma unroll 1
+    for (int pass = 0; pass < nPass; ++pass) {
+        const int r = pass * kLanes + lane;
+        if (r >= mr.nR) continue;
+        const long o = (long)r * kWidth;
+        const int nl = mr.nl[r], nr = mr.nr[r];
+        adj::RxnBarT<kTeamWidth> in;
+        in.fast = true;
+        auto u = [&](int col) {
+            double v = 0.0;
+            for (int q = 0; q < kTeamWidth && q < nl; ++q) v -= mr.ls[o + q] * L.Jb[mr.li[o + q] * P + col];
+            for (int q = 0; q < kTeamWidth && q < nr; ++q) v += mr.rs[o + q] * L.Jb[mr.ri[o + q] * P + col];
+            return v;
+        };
+        for (int q = 0; q < kTeamWidth && q < nl; ++q) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
+        for (int q = 0; q < kTeamWidth && q < nr; ++q) in.w += mr.rs[o + q] * L.dNb[mr.ri[o + q]];
+        for (int j = 0; j < kTeamWidth && j < nl; ++j) in.dCf[j] = u(mr.li[o + j]);
+        for (int j = 0; j < kTeamWidth && j < nr; ++j) in.dCr[j] = -u(mr.ri[o + j]);
+        in.dwdT = u(iT);
+        if (mr.type[r] == GFC_RATE_THIRD_BODY) {
+            const double* eff = mr.eff + (long)r * nS;
+            double D = 0.0;
+            for (int j = 0; j < nS; ++j)
+                if (eff[j] != 0.0) D += eff[j] * u(j);
+            in.D = D;
+        }
+        double t6[adj::kParams] = {0, 0, 0, 0, 0, 0};
+        adj::reaction_reverse_w<kTeamWidth>(mr, r, T, false, lane::SV{L.f.c, 1}, in,
+                                            adj::WaveAcc{L.cb}, Tb, t6);
+        th.add(mr.rxnRef[r], t6);
+    }
+    Tb = wave_sum(Tb);
+    wave_sync();
+    const double cb = sp ? L.cb[lane] : 0.0;
+    w_mixture_reverse(mr, p, y, T, x, cb, Cpb, Hab, vb, true, mb, Tb, ybar, lane);
+    wave_sync();
+}
+
+// Reverse of one accepted step of size h from y0 (adjoint_math.hpp
+// step_reverse): takes this lane's component of the adjoint of the step's
+// result and returns that of y0.
+template <int N>
+__device__ __forceinline__ double w_step_reverse(const SolveArgs& A, const DeviceMech& mr,
+                                                 WaveAdjLDS<N>& L, double p, double y0,
+                                                 double h, double yb, const adj::ThView& th,
+                                                 int lane) {
+    const DeviceMech& m = A.m;
+    const int nS = m.nS, n = nS + 1;
+    const RosTableau& t = A.tab;
+    const double T0 = readlane(y0, nS);
+    const bool inRange = T0 >= m.Tlow && T0 <= m.Thigh;
+    WStages S;
+    (void)w_step<N, true>(A, L, p, y0, h, lane, S);
+    double kb1 = t.b1 * yb, kb2 = t.b2 * yb, kb3 = t.b3 * yb, kb4 = t.b4 * yb;
+    double Fb = 0.0, fb2 = 0.0, fb3 = 0.0;
+    for (int e = lane; e < N * WaveAdjLDS<N>::P; e += kLanes) L.Jb[e] = 0.0;
+    wave_sync();
+    if (t.stages == 4) {
+        const double rb = w_solveT<N>(L, n, S.ps, kb4, lane);
+        w_add_outer<N>(L, n, rb, S.k4, lane);
+        fb3 += rb;
+        Fb += h * t.d4 * rb;
+        kb1 += t.c41 / h * rb;
+        kb2 += t.c42 / h * rb;
+        kb3 += t.c43 / h * rb;
+    }
+    if (t.stages >= 3) {
+        const double rb = w_solveT<N>(L, n, S.ps, kb3, lane);
+        w_add_outer<N>(L, n, rb, S.k3, lane);
+        if (t.stages == 4) fb3 += rb;
+        else fb2 += rb;
+        Fb += h * t.d3 * rb;
+        kb1 += t.c31 / h * rb;
+        kb2 += t.c32 / h * rb;
+    }
+    if (t.stages == 4) {
+        const double ys = y0 + t.a31 * S.k1 + t.a32 * S.k2;
+        double ysb = 0.0;
+        w_vjp_f<N>(mr, L, p, ys, fb3, S.f3, ysb, th, lane);
+        yb += ysb;
+        kb1 += t.a31 * ysb;
+        kb2 += t.a32 * ysb;
+    }
+    {
+        const double rb = w_solveT<N>(L, n, S.ps, kb2, lane);
+        w_add_outer<N>(L, n, rb, S.k2, lane);
+        fb2 += rb;
+        Fb += h * t.d2 * rb;
+        kb1 += t.c21 / h * rb;
+    }
+    {
+        const double ys = y0 + t.a21 * S.k1;
+        double ysb = 0.0;
+        w_vjp_f<N>(mr, L, p, ys, fb2, S.f2, ysb, th, lane);
+        yb += ysb;
+        kb1 += t.a21 * ysb;
+    }
+    const double rb = w_solveT<N>(L, n, S.ps, kb1, lane);
+    w_add_outer<N>(L, n, rb, S.k1, lane);
+    Fb += h * t.d1 * rb;
+    // within the reaction temperature limits f(y0) is the Jacobian call's F
+    if (inRange) Fb += rb;
+    else w_vjp_f<N>(mr, L, p, y0, rb, S.dydx0, yb, th, lane);
+    w_jact<N>(mr, L, p, y0, S.F, Fb, yb, th, lane);
+    return yb;
+}
+
+} // namespace
+
+// Reverse sweep, one wavefront per cell (adjoint.hip adjoint_reverse_kernel).
+// A.m: the wavefront solve's tables.  seg: the K-state segment buffers
+// (segPer doubles per cell, wave-blocked).  Ybar/Y0bar: [n][ldb]; thBar:
+// [6 nR][ldb], reactions in reference order.
+template <int N>
+__global__ void __launch_bounds__(64)
+wave_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer,
+                            const double* Ybar, long ldb, double* Y0bar, double* thBar,
+                            int* queue) {
+    __shared__ WaveAdjLDS<N> L;
+    const int lane = threadIdx.x;
+    const DeviceMech& mr = A.m;
+    const int nS = A.m.nS, n = nS + 1;
+    for (;;) {
+        int c = 0;
+        if (lane == 0) c = atomicAdd(queue, 1);
+        const int cell = __builtin_amdgcn_readfirstlane(__shfl(c, 0));
+        if (cell >= A.nCells) break;
+        const double* y0c = rec_slot(R.y0c, cell, n + 1);
+        const double p = y0c[(long)n * kRecStride];
+        const double T0 = y0c[(long)nS * kRecStride];
+        double yb = (lane < n) ? Ybar[lane * ldb + cell] : 0.0;
+        const adj::ThView th{thBar ? thBar + cell : nullptr, ldb};
+        if (th.p)
+            for (long q = lane; q < (long)adj::kParams * mr.nR; q += kLanes) th.p[q * ldb] = 0.0;
+        if (T0 > A.Treact && A.deltaT > kSmall) {
+            const double* yE = rec_slot(R.yEnd, cell, n);
+            if (lane < nS && yE[lane * kRecStride] < 0.0) yb = 0.0;
+            const int nAcc = R.nAcc[cell];
+            const double* ck = rec_slot(R.ck, cell, R.nck * n);
+            const double* hs = rec_slot(R.hs, cell, R.maxSteps);
+            double* sg = rec_slot(seg, cell, segPer);
+            const int nseg = (nAcc + R.K - 1) / R.K;
+            for (int g = nseg - 1; g >= 0; --g) {
+                const int s0 = g * R.K, s1 = min(s0 + R.K, nAcc);
+                double y = (lane < n) ? ck[((long)g * n + lane) * kRecStride] : 0.0;
+                if (lane < n) sg[lane * kRecStride] = y;
+                WStages S;
+                for (int t = s0; t < s1 - 1; ++t) {
+                    y = w_step<N, false>(A, L, p, y, hs[(long)t * kRecStride], lane, S);
+                    if (lane < n) sg[((long)(t - s0 + 1) * n + lane) * kRecStride] = y;
+                }
+                for (int t = s1 - 1; t >= s0; --t) {
+                    const double y0 = (lane < n) ? sg[((long)(t - s0) * n + lane) * kRecStride] : 0.0;
+                    yb = w_step_reverse<N>(A, mr, L, p, y0, hs[(long)t * kRecStride], yb, th,
+                                           lane);
+                }
+            }
+        }
+        if (lane < n) Y0bar[lane * ldb + cell] = yb;
+        wave_sync();
+    }
+}
+
+template __global__ void wave_adjoint_reverse_kernel<32>(SolveArgs, T16Record, double*, long,
+                                                         const double*, long, double*, double*,
+                                                         int*);
+template __global__ void wave_adjoint_reverse_kernel<54>(SolveArgs, T16Record, double*, long,
+                                                         const double*, long, double*, double*,
+                                                         int*);
+template __global__ void wave_adjoint_reverse_kernel<64>(SolveArgs, T16Record, double*, long,
+                                                         const double*, long, double*, double*,
+                                                         int*);
+
+} // namespace gfc

@@ -281,8 +281,11 @@ int gfc_solve_noalloc(gfc_evaluator* ev, double deltaT, double deltaTChemMax,
  * per cell up to 208, else one lane per cell), 1 = one wavefront per cell,
  * 4 = four 16-lane cells per wavefront (nSpecie+1 <= 16, at most 16
  * third-body efficiency sets, at most 3 species per reaction side; other
- * mechanisms return GFC_ERR_INVALID_ARGUMENT), 64 = one lane per cell.
- * wavesPerBlock: resident wavefronts per compute unit of the persistent
+ * mechanisms return GFC_ERR_INVALID_ARGUMENT), 64 = one lane per cell,
+ * 2 = one cell per workgroup of two wavefronts (32 < nSpecie+1 <= 64, at
+ * most 16 efficiency sets, at most 3 species per reaction side; others
+ * return GFC_ERR_INVALID_ARGUMENT).
+ * wavesPerBlock: resident workgroups per compute unit of the persistent
  * grid (0 = as many as fit).  chunk: cells per work-queue ticket of the
  * wavefront kernel: 0 = automatic (16 consecutive cells, one 128-byte line
  * of every state row, per ticket, except the last 16 per wavefront, which
@@ -318,6 +321,7 @@ int gfc_evaluator_get_factorization(const gfc_evaluator* ev, int* variant);
  * LUDecompose + LUBacksubstitute, ludecompose.H:9-140; tested in
  * unittest/tests/Test-gpuChemistry.C:521-691).  shape: 0 = the family the
  * batched solve uses for this evaluator, GFC_SHAPE_WAVE (nSpecie+1 <= 64),
+ * GFC_SHAPE_PAIR (two wavefronts per cell, 32 < nSpecie+1 <= 64),
  * GFC_SHAPE_TEAM16 (nSpecie+1 <= 16), GFC_SHAPE_WORKGROUP
  * (64 < nSpecie+1 <= 208); a family that cannot take the mechanism, or a
  * mechanism with rate types beyond gpuFoam's four, returns
@@ -332,6 +336,7 @@ int gfc_evaluator_get_factorization(const gfc_evaluator* ev, int* variant);
  * the workgroup family, the evaluator's factorisation variant) and solves
  * for b ([n][ldB]) into x ([n][ldB]). */
 #define GFC_SHAPE_WAVE 1
+#define GFC_SHAPE_PAIR 2
 #define GFC_SHAPE_TEAM16 4
 #define GFC_SHAPE_WORKGROUP 512
 int gfc_eval_derivatives(gfc_evaluator* ev, int shape, int nCells, int ldY, const double* Y,
@@ -398,7 +403,10 @@ int gfc_evaluator_set_rate_parameters(gfc_evaluator* ev, const double* params);
  * no deltaTMin, deltaTChem not clamped) that records the accepted steps.
  * For nSpecie+1 <= 16 it takes the batched form's choice of four cells per
  * wavefront (gfc_evaluator_set_batched_shape) and then returns the batched
- * solve's results; otherwise it returns the one-lane solve's
+ * solve's results; for 16 < nSpecie+1 <= 64 it runs one wavefront per cell
+ * (the batched wavefront kernel with the recorder compiled in: the batched
+ * solve's results) unless the batched shape asks for one lane per cell
+ * (cellsPerWave 64); otherwise it returns the one-lane solve's
  * (GFC_LANE_PER_CELL).  The workspace
  * (step sizes, checkpoints every K accepted steps, per-cell scratch) is
  * allocated by this call when it grows.

@@ -1,8 +1,11 @@
-"""GPU tests of the discrete adjoint through the C ABI: the recording
-forward is the one-lane solve, the device gradients equal the host build of
-the same adjoint code (itself checked against the oracle on the CPU), and
-they match central finite differences of the replayed accepted steps
-(relative step 1e-6), also through the test-only torch.autograd.Function."""
+"""GPU tests of the discrete adjoint through the C ABI, one lane per cell
+(the default for these H2 batch sizes and for n > 64; set_batched_shape(64)
+for GRI, whose default is one wavefront per cell, tests/test_gpu_adjoint_wave.py):
+the recording forward is the one-lane solve, the device gradients equal the
+host build of the same adjoint code (itself checked against the oracle on
+the CPU), and they match central finite differences of the replayed
+accepted steps (relative step 1e-6), also through the test-only
+torch.autograd.Function."""
 import numpy as np
 import pytest
 
@@ -34,6 +37,12 @@ def _case(config, ncells, maxSteps=None):
     return m, Y0, dt0, c, ode
 
 
+def _lane_evaluator(m, ode, ncells):
+    ev = Evaluator(m, ode, 0.0, max_cells=ncells)
+    ev.set_batched_shape(64)  # one lane per cell for every mechanism
+    return ev
+
+
 def _forward(ev, c, Y0, dt0):
     dY0 = torch.tensor(Y0, device=DEV)
     dYn = torch.empty_like(dY0)
@@ -46,7 +55,7 @@ def _forward(ev, c, Y0, dt0):
 @pytest.mark.parametrize("config", ["h2_small", "gri_production"])
 def test_recording_forward_is_the_plain_solve(config):
     m, Y0, dt0, c, ode = _case(config, 96)
-    ev = Evaluator(m, ode, 0.0, max_cells=96)
+    ev = _lane_evaluator(m, ode, 96)
     _, dYn, ddt, st = _forward(ev, c, Y0, dt0)
     Yn, dt, _, stp, _ = ev.solve_host(c["deltaT"], Y0, dt0, lane=True)
     torch.cuda.synchronize()
@@ -59,7 +68,7 @@ def test_recording_forward_is_the_plain_solve(config):
 def test_device_gradients_equal_host_build(config, ncells):
     m, Y0, dt0, c, ode = _case(config, ncells)
     nS, n = m.n_species, m.n_species + 1
-    ev = Evaluator(m, ode, 0.0, max_cells=ncells)
+    ev = _lane_evaluator(m, ode, ncells)
     _, _, _, st = _forward(ev, c, Y0, dt0)
     rng = np.random.default_rng(0)
     W = rng.standard_normal((n, ncells))
@@ -90,7 +99,7 @@ def test_device_gradients_equal_host_build(config, ncells):
 def test_gradients_match_replayed_central_differences(config, ncells, maxSteps):
     m, Y0, dt0, c, ode = _case(config, ncells, maxSteps)
     nS, n, nR = m.n_species, m.n_species + 1, m.n_reactions
-    ev = Evaluator(m, ode, 0.0, max_cells=ncells)
+    ev = _lane_evaluator(m, ode, ncells)
     theta0 = ev.get_rate_parameters()
     ev.set_rate_parameters(theta0)
     dY0, _, _, _ = _forward(ev, c, Y0, dt0)

@@ -23,8 +23,9 @@ held to the oracle at the same states:
   and the scaled residual |M x - b| / (max|M| max|x|) <= 1e-12.
 
 Mechanisms: H2 (n = 11: the wavefront and 16-lane kernels), GRI-Mech 3.0
-(n = 54: wavefront), a 60-species synthetic mechanism (n = 61: the 64-row
-wavefront kernel), and the 100- and 200-species synthetic ones (n = 101,
+(n = 54: wavefront, and two wavefronts per cell), a 60-species synthetic
+mechanism (n = 61: the 64-row wavefront kernel), a 60-species one with 45
+reactions (the padded 64-row two-wavefront kernel), and the 100- and 200-species synthetic ones (n = 101,
 201: the two workgroup register blocks, every factorisation variant).  No
 bundled mechanism has 16 < n <= 32, the 32-row wavefront instantiation's
 range (the synthetic generator needs every GRI species).
@@ -64,6 +65,11 @@ def _case(name):
     elif name == "syn60":
         m = _syn(60, 300, 23)
         Y0, _ = syn_states(m, 24, seed=24)
+    elif name == "syn60s":
+        # 45 reactions, 12 efficiency sets: the padded two-wavefront kernel
+        t, r = syn_texts(seed=13, nS=60, nR=45)
+        m = Mechanism(t, r, "syn60s")
+        Y0, _ = syn_states(m, 16, seed=14)
     elif name == "syn100":
         m = _syn(100, 600, 7)
         Y0, _ = syn_states(m, 16, seed=11)
@@ -77,7 +83,7 @@ def _case(name):
 
 
 SHAPES = [("h2", _capi.SHAPE_WAVE), ("h2", _capi.SHAPE_TEAM16), ("gri", _capi.SHAPE_WAVE),
-          ("syn60", _capi.SHAPE_WAVE),
+          ("gri", _capi.SHAPE_PAIR), ("syn60", _capi.SHAPE_WAVE), ("syn60s", _capi.SHAPE_PAIR),
           ("syn100", _capi.SHAPE_WORKGROUP), ("syn200", _capi.SHAPE_WORKGROUP)]
 
 

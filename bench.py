@@ -204,6 +204,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cells-per-wave", type=int, choices=[0, 1, 2, 4, 64], default=0,
+                    help="launch shape of the batched solve: 0 automatic, 1 a wavefront per "
+                         "cell, 2 two wavefronts per cell, 4 four cells per wavefront, 64 one "
+                         "lane per cell")
     ap.add_argument("--queue-chunk", type=int, choices=[0, 1], default=0,
                     help="cells per work-queue ticket of the wavefront kernel: 0 automatic "
                          "(16-cell chunks), 1 one cell per ticket")
@@ -256,8 +260,8 @@ def main():
     dmin = torch.zeros(1, dtype=torch.float64, device=dev)
     stats = torch.zeros((3, n), dtype=torch.int32, device=dev)
     ev = Evaluator(m, ode, 0.0, max_cells=n)
-    if args.queue_chunk:
-        ev.set_batched_shape(0, 0, args.queue_chunk)
+    if args.queue_chunk or args.cells_per_wave:
+        ev.set_batched_shape(args.cells_per_wave, 0, args.queue_chunk)
     if args.factorization:
         ev.set_factorization({"column": _capi.LU_COLUMN, "panel": _capi.LU_PANEL,
                               "mfma": _capi.LU_PANEL_MFMA}[args.factorization])
@@ -378,7 +382,8 @@ def main():
                    "solver": c["solver"], "relTol": c["relTol"], "absTol": c["absTol"],
                    "deltaT": c["deltaT"], "graph": graph,
                    "factorization": args.factorization or "default",
-                   "queue_chunk": args.queue_chunk},
+                   "queue_chunk": args.queue_chunk,
+                   "cells_per_wave": args.cells_per_wave or "automatic"},
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": traffic,

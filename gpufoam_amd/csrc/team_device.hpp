@@ -127,8 +127,10 @@ __device__ __forceinline__ double team_derivatives(const DeviceMech& m, double p
 // gpuODESystem::jacobian (gpuODESystem.H:198-281).  Leaves J (n x n, the
 // pressure row/column dropped) in L.J and returns this lane's component of
 // the derivative computed alongside (the reference's "dfdx").  With RAW,
-// the species rows before the transform (dN_i/dc_j with the dense
-// third-body part, and dN_i/dT) are also stored to raw[i*P + j].
+// two quantities of each species row before the transform (dN_i/dc_j with
+// the dense third-body part, and dN_i/dT) are also stored:
+// raw[i] = sum_j (dN_i/dc_j) c_j, summed in column order, and
+// raw[64 + i] = dN_i/dT.
 template <int N, bool RAW = false>
 __device__ __forceinline__ double team_jacobian(const DeviceMech& m, double p, double T,
                                              double yl, TeamLDS<N>& L, int lane,
@@ -307,7 +309,8 @@ __device__ __forceinline__ double team_jacobian(const DeviceMech& m, double p, d
         double* row = &L.J[lane * P];
         const double WiByrhoM = m.W[lane] / rhoM;
         double ddNdT = row[nS];
-        if constexpr (RAW) raw[lane * P + nS] = ddNdT;
+        double sumc = 0.0;
+        if constexpr (RAW) raw[kLanes + lane] = ddNdT;
         if (fusedDense) {
             // this row's weight of every set; the sets' efficiencies of
             // column j are uniform.  Column j+1's efficiencies are in flight
@@ -325,7 +328,7 @@ __device__ __forceinline__ double team_jacobian(const DeviceMech& m, double p, d
                 dense_row_load(m.effSetsT + min(j + 1, nS - 1) * kSetsT, en);
                 const double acc = dense_dot(w, e);
                 const double tj = row[j] + acc;
-                if constexpr (RAW) raw[lane * P + j] = tj;
+                if constexpr (RAW) sumc += tj * L.c[j];
                 ddNdT -= tj * L.c[j] * alphavM;
                 row[j] = WiByrhoM * tj * L.colA[j] + L.colB[j] * d;
             }
@@ -333,12 +336,13 @@ __device__ __forceinline__ double team_jacobian(const DeviceMech& m, double p, d
 #pragma unroll 4
             for (int j = 0; j < nS; ++j) {
                 const double tj = row[j];
-                if constexpr (RAW) raw[lane * P + j] = tj;
+                if constexpr (RAW) sumc += tj * L.c[j];
                 ddNdT -= tj * L.c[j] * alphavM;
                 row[j] = WiByrhoM * tj * L.colA[j] + L.colB[j] * d;
             }
         }
         row[nS] = WiByrhoM * ddNdT + alphavM * d;
+        if constexpr (RAW) raw[lane] = sumc;
     }
     __syncthreads();
     // ddT_dtdY (:108-126): lane i sums column i

@@ -9,10 +9,14 @@
 // (team_device.hpp), then applies the reverse of the step as
 // adjoint_math.hpp's step_reverse does:
 //   * lane i holds component i of every vector;
-//   * the step's Jacobian (transformed and, in raw, its species rows before
-//     the transform), its LU factors (rows in pivot order) and the adjoint
-//     of the Jacobian (Jb) live in LDS;
-//   * transposed solves read the LU from LDS, one position per broadcast;
+//   * the step's Jacobian (after the transform) and the adjoint of the
+//     Jacobian (Jb) live in LDS; of the species rows before the transform
+//     the Jacobian adjoint needs dN_i/dT, each row's product with c (both
+//     kept when the Jacobian is assembled) and one column combination,
+//     recovered from the transformed rows;
+//   * the LU factors go through the Jb tile once, rows in pivot order, so
+//     that lane k keeps column k of them in registers; the transposed
+//     solves then read their entries from registers;
 //   * reaction sweeps run 64 reactions per pass, one per lane, with the
 //     one-lane form's per-reaction reverse (reaction_reverse_w) and LDS
 //     atomics for the concentration adjoints;
@@ -20,8 +24,8 @@
 //     sequential order in the last bits.
 // The sweeps use the wavefront solve's tables, reactions grouped by rate
 // type; rate-parameter adjoints go to each reaction's reference index
-// (rxnRef).  The LDS of a cell (four n x n tiles) holds one wavefront per
-// compute unit for GRI-Mech's n = 54.
+// (rxnRef).  The LDS of a cell (two n x (n+1) tiles and the solve's arrays,
+// 65 KB for GRI-Mech's n = 54) holds two wavefronts per compute unit.
 #include <hip/hip_runtime.h>
 
 #include "adjoint_math.hpp"
@@ -49,9 +53,9 @@ template <int N>
 struct WaveAdjLDS {
     static constexpr int P = TeamLDS<N>::P;
     TeamLDS<N> f;       // the solve's arrays; f.J holds the step's Jacobian
-    double raw[N * P];  // species rows before the transform
-    double LU[N * P];   // LU factors, row j = pivot position j
-    double Jb[N * P];   // adjoint of the Jacobian
+    double Jb[N * P];   // adjoint of the Jacobian (first the LU's transpose)
+    double raw[2 * kLanes]; // per species row before the transform: sum_j
+                            // (dN_i/dc_j) c_j, then dN_i/dT
     double cb[kLanes];  // concentration adjoints
     double dNb[kLanes]; // adjoints of dN_i/dt
     double vj[kLanes];  // 1/rho_i of each specie
@@ -65,17 +69,20 @@ __device__ __forceinline__ double bcast_from(double v, int src) {
 }
 
 // One step's recomputed quantities, this lane's components.
+template <int N>
 struct WStages {
     double F, dydx0, k1, k2, k3, k4, f2, f3;
-    int ps; // pivot position of this lane's row
+    int ps;       // pivot position of this lane's row
+    double col[N]; // column `lane` of the LU factors, rows in pivot order
 };
 
 // The accepted attempt of a step of size dx from y, as the solve computes it
 // (team_kernel.hip): returns the new state.  With KEEP, also leaves the
-// pre-transform rows in L.raw and the LU in L.LU.
+// pre-transform row quantities in L.raw and this lane's column of the LU
+// factors in S.col.
 template <int N, bool KEEP>
 __device__ __forceinline__ double w_step(const SolveArgs& A, WaveAdjLDS<N>& L, double p,
-                                         double y, double dx, int lane, WStages& S) {
+                                         double y, double dx, int lane, WStages<N>& S) {
     constexpr int P = WaveAdjLDS<N>::P;
     const DeviceMech& m = A.m;
     const int nS = m.nS, n = nS + 1;
@@ -99,12 +106,18 @@ __device__ __forceinline__ double w_step(const SolveArgs& A, WaveAdjLDS<N>& L, d
     double rd;
     team_lu<N>(row, n, lane, ps, rd, plv);
     if constexpr (KEEP) {
-        // (a NaN Jacobian can leave a row at a padding position >= n)
+        // rows to pivot positions in the Jb tile, then each lane takes its
+        // column (a NaN Jacobian can leave a row at a padding position >= n)
         if (lane < n && ps < n) {
 #pragma unroll
             for (int k = 0; k < N; ++k)
-                if (k < n) L.LU[ps * P + k] = row[k];
+                if (k < n) L.Jb[ps * P + k] = row[k];
         }
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            S.col[j] = (j < n && lane < n) ? L.Jb[j * P + min(lane, N - 1)] : 0.0;
+        wave_sync();
     }
     double k1 = 0, k2 = 0, k3 = 0, k4 = 0, dydx = 0, f2 = 0, f3 = 0;
 #pragma unroll 1
@@ -142,24 +155,28 @@ __device__ __forceinline__ double w_step(const SolveArgs& A, WaveAdjLDS<N>& L, d
 }
 
 // Solves A^T x = r for the step's A = P^T L U (adjoint_math.hpp lu_solve_T):
-// lane i holds r_i and receives x_i.
+// lane i holds r_i and receives x_i; lane k holds column k of the factors
+// (col[j] = LU[j][k], rows in pivot order).
 template <int N>
-__device__ __forceinline__ double w_solveT(const WaveAdjLDS<N>& L, int n, int ps, double r,
+__device__ __forceinline__ double w_solveT(const double (&col)[N], int n, int ps, double r,
                                            int lane) {
-    constexpr int P = WaveAdjLDS<N>::P;
     // U^T w = r
     double w = 0.0;
-    for (int j = 0; j < n; ++j) {
-        const double wj = readlane(r, j) / L.LU[j * P + j];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        if (j >= n) break;
+        const double wj = readlane(r, j) / readlane(col[j], j);
         if (lane == j) w = wj;
-        if (lane > j && lane < n) r -= L.LU[j * P + lane] * wj;
+        if (lane > j && lane < n) r -= col[j] * wj;
     }
     // L^T v = w (unit diagonal)
     double v = 0.0;
-    for (int k = n - 1; k >= 0; --k) {
+#pragma unroll
+    for (int k = N - 1; k >= 0; --k) {
+        if (k >= n) continue;
         const double vk = readlane(w, k);
         if (lane == k) v = vk;
-        if (lane < k) w -= L.LU[k * P + lane] * vk;
+        if (lane < k) w -= col[k] * vk;
     }
     // x = P^T v: row i sits at position ps_i
     return bcast_from(v, ps);
@@ -288,7 +305,7 @@ __device__ __forceinline__ void w_vjp_f(const DeviceMech& mr, WaveAdjLDS<N>& L, 
 }
 
 // ybar += d/dy and th += d/dtheta of <Jb, J(y)> + <Fb, F(y)> for the fast
-// Jacobian J = L.f.J (pre-transform rows in L.raw) and F at y
+// Jacobian J = L.f.J (pre-transform row quantities in L.raw) and F at y
 // (adjoint_math.hpp jact).  Jb is consumed.
 template <int N>
 __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, double p,
@@ -347,11 +364,16 @@ __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, d
     // row transforms: vb_j (species columns) and the temperature column's
     // concentration adjoints.
     if (sp) {
+        // sum_i g_i (W_i/rho) t_ij with t the rows before the transform:
+        // the transform made J_ij = (W_i/rho) t_ij (rho/W_j) + rho v_j F_i
+        double gJ = 0.0, gF = 0.0;
         for (int i = 0; i < nS; ++i) {
             vb += L.Jb[i * P + lane] * rho * L.v[i];
             const double g = L.Jb[i * P + iT];
-            cbT += -g * (mr.W[i] / rho) * L.raw[i * P + lane] * av;
+            gJ += g * L.f.J[i * P + lane];
+            gF += g * L.v[i];
         }
+        cbT = -(gJ - (rho * x.vi) * gF) / (rho / mr.W[lane]) * av;
     }
     wave_sync();
     double rhoPart = 0.0, avPart = 0.0;
@@ -366,9 +388,8 @@ __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, d
         }
         // temperature column
         const double g = row[iT];
-        double sumc = 0.0;
-        for (int j = 0; j < nS; ++j) sumc += L.raw[lane * P + j] * L.f.c[j];
-        const double X = L.raw[lane * P + iT] - sumc * av;
+        const double sumc = L.raw[lane];
+        const double X = L.raw[kLanes + lane] - sumc * av;
         row[iT] = g * WiR;
         for (int j = 0; j < nS; ++j) row[j] += -g * WiR * L.f.c[j] * av;
         avPart = g * (F - WiR * sumc);
@@ -449,14 +470,14 @@ __device__ __forceinline__ double w_step_reverse(const SolveArgs& A, const Devic
     const RosTableau& t = A.tab;
     const double T0 = readlane(y0, nS);
     const bool inRange = T0 >= m.Tlow && T0 <= m.Thigh;
-    WStages S;
+    WStages<N> S;
     (void)w_step<N, true>(A, L, p, y0, h, lane, S);
     double kb1 = t.b1 * yb, kb2 = t.b2 * yb, kb3 = t.b3 * yb, kb4 = t.b4 * yb;
     double Fb = 0.0, fb2 = 0.0, fb3 = 0.0;
     for (int e = lane; e < N * WaveAdjLDS<N>::P; e += kLanes) L.Jb[e] = 0.0;
     wave_sync();
     if (t.stages == 4) {
-        const double rb = w_solveT<N>(L, n, S.ps, kb4, lane);
+        const double rb = w_solveT<N>(S.col, n, S.ps, kb4, lane);
         w_add_outer<N>(L, n, rb, S.k4, lane);
         fb3 += rb;
         Fb += h * t.d4 * rb;
@@ -465,7 +486,7 @@ __device__ __forceinline__ double w_step_reverse(const SolveArgs& A, const Devic
         kb3 += t.c43 / h * rb;
     }
     if (t.stages >= 3) {
-        const double rb = w_solveT<N>(L, n, S.ps, kb3, lane);
+        const double rb = w_solveT<N>(S.col, n, S.ps, kb3, lane);
         w_add_outer<N>(L, n, rb, S.k3, lane);
         if (t.stages == 4) fb3 += rb;
         else fb2 += rb;
@@ -482,7 +503,7 @@ __device__ __forceinline__ double w_step_reverse(const SolveArgs& A, const Devic
         kb2 += t.a32 * ysb;
     }
     {
-        const double rb = w_solveT<N>(L, n, S.ps, kb2, lane);
+        const double rb = w_solveT<N>(S.col, n, S.ps, kb2, lane);
         w_add_outer<N>(L, n, rb, S.k2, lane);
         fb2 += rb;
         Fb += h * t.d2 * rb;
@@ -495,7 +516,7 @@ __device__ __forceinline__ double w_step_reverse(const SolveArgs& A, const Devic
         yb += ysb;
         kb1 += t.a21 * ysb;
     }
-    const double rb = w_solveT<N>(L, n, S.ps, kb1, lane);
+    const double rb = w_solveT<N>(S.col, n, S.ps, kb1, lane);
     w_add_outer<N>(L, n, rb, S.k1, lane);
     Fb += h * t.d1 * rb;
     // within the reaction temperature limits f(y0) is the Jacobian call's F
@@ -544,7 +565,7 @@ wave_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer,
                 const int s0 = g * R.K, s1 = min(s0 + R.K, nAcc);
                 double y = (lane < n) ? ck[((long)g * n + lane) * kRecStride] : 0.0;
                 if (lane < n) sg[lane * kRecStride] = y;
-                WStages S;
+                WStages<N> S;
                 for (int t = s0; t < s1 - 1; ++t) {
                     y = w_step<N, false>(A, L, p, y, hs[(long)t * kRecStride], lane, S);
                     if (lane < n) sg[((long)(t - s0 + 1) * n + lane) * kRecStride] = y;

@@ -9,14 +9,20 @@
 // (team_device.hpp), then applies the reverse of the step as
 // adjoint_math.hpp's step_reverse does:
 //   * lane i holds component i of every vector;
-//   * the step's Jacobian (after the transform) and the adjoint of the
-//     Jacobian (Jb) live in LDS; of the species rows before the transform
-//     the Jacobian adjoint needs dN_i/dT, each row's product with c (both
-//     kept when the Jacobian is assembled) and one column combination,
-//     recovered from the transformed rows;
-//   * the LU factors go through the Jb tile once, rows in pivot order, so
-//     that lane k keeps column k of them in registers; the transposed
-//     solves then read their entries from registers;
+//   * the step's Jacobian (after the transform) lives in LDS; of the
+//     species rows before the transform the Jacobian adjoint needs dN_i/dT,
+//     each row's product with c (both kept when the Jacobian is assembled)
+//     and one column combination, recovered from the transformed rows;
+//   * the adjoint of the Jacobian (Jb) is kept as factors: it is the sum of
+//     the stages' outer products r_s k_s^T, and the reverse of the
+//     Jacobian's transforms adds three rank-one terms and scales rows and
+//     columns, so Jb = sum_q a_q b_q^T with at most seven terms, held in the
+//     region of the solve's dense third-body weights (dead once the
+//     Jacobian is built); every Jb entry or row / column sum the reverse
+//     needs is formed from the factors;
+//   * the LU factors go through that region once, 16 rows in pivot order at
+//     a time, so that lane k keeps column k of them in registers; the
+//     transposed solves read their entries from registers;
 //   * reaction sweeps run 64 reactions per pass, one per lane, with the
 //     one-lane form's per-reaction reverse (reaction_reverse_w) and LDS
 //     atomics for the concentration adjoints;
@@ -24,8 +30,8 @@
 //     sequential order in the last bits.
 // The sweeps use the wavefront solve's tables, reactions grouped by rate
 // type; rate-parameter adjoints go to each reaction's reference index
-// (rxnRef).  The LDS of a cell (two n x (n+1) tiles and the solve's arrays,
-// 65 KB for GRI-Mech's n = 54) holds two wavefronts per compute unit.
+// (rxnRef).  The LDS of a cell (the solve's arrays and five vectors, 40 KB
+// for GRI-Mech's n = 54) holds four wavefronts per compute unit.
 #include <hip/hip_runtime.h>
 
 #include "adjoint_math.hpp"
@@ -49,11 +55,15 @@ using t16::rec_slot;
 using t16::T16Record;
 using t16::wave_sync;
 
+// Factors of the Jacobian's adjoint: Jb = sum_{q < rank} a_q b_q^T.
+constexpr int kJbRank = 8;
+static_assert(2 * kJbRank * kLanes <= kMaxSets * kLanes, "Jb factors in the weight region");
+
 template <int N>
 struct WaveAdjLDS {
     static constexpr int P = TeamLDS<N>::P;
-    TeamLDS<N> f;       // the solve's arrays; f.J holds the step's Jacobian
-    double Jb[N * P];   // adjoint of the Jacobian (first the LU's transpose)
+    TeamLDS<N> f;       // the solve's arrays; f.J holds the step's Jacobian,
+                        // f.Wd the LU's transpose, then the factors of Jb
     double raw[2 * kLanes]; // per species row before the transform: sum_j
                             // (dN_i/dc_j) c_j, then dN_i/dT
     double cb[kLanes];  // concentration adjoints
@@ -106,18 +116,26 @@ __device__ __forceinline__ double w_step(const SolveArgs& A, WaveAdjLDS<N>& L, d
     double rd;
     team_lu<N>(row, n, lane, ps, rd, plv);
     if constexpr (KEEP) {
-        // rows to pivot positions in the Jb tile, then each lane takes its
-        // column (a NaN Jacobian can leave a row at a padding position >= n)
-        if (lane < n && ps < n) {
+        // BR pivot positions at a time: their rows into the weight region,
+        // then each lane takes its column (a NaN Jacobian can leave a row at
+        // a padding position >= n)
+        double* X = L.f.Wd;
+        constexpr int BR = kMaxSets * kLanes / P;
 #pragma unroll
-            for (int k = 0; k < N; ++k)
-                if (k < n) L.Jb[ps * P + k] = row[k];
+        for (int b = 0; b < (N + BR - 1) / BR; ++b) {
+            if (lane < n && ps < n && ps >= BR * b && ps < BR * b + BR) {
+#pragma unroll
+                for (int k = 0; k < N; ++k)
+                    if (k < n) X[(ps - BR * b) * P + k] = row[k];
+            }
+            wave_sync();
+#pragma unroll
+            for (int t = 0; t < BR; ++t) {
+                const int j = BR * b + t;
+                if (j < N) S.col[j] = (j < n && lane < n) ? X[t * P + min(lane, N - 1)] : 0.0;
+            }
+            wave_sync();
         }
-        wave_sync();
-#pragma unroll
-        for (int j = 0; j < N; ++j)
-            S.col[j] = (j < n && lane < n) ? L.Jb[j * P + min(lane, N - 1)] : 0.0;
-        wave_sync();
     }
     double k1 = 0, k2 = 0, k3 = 0, k4 = 0, dydx = 0, f2 = 0, f3 = 0;
 #pragma unroll 1
@@ -182,16 +200,31 @@ __device__ __forceinline__ double w_solveT(const double (&col)[N], int n, int ps
     return bcast_from(v, ps);
 }
 
-// Jb(i, j) += r_i k_j
+template <int N>
+__device__ __forceinline__ double* jb_a(WaveAdjLDS<N>& L, int q) {
+    return L.f.Wd + q * kLanes;
+}
+template <int N>
+__device__ __forceinline__ double* jb_b(WaveAdjLDS<N>& L, int q) {
+    return L.f.Wd + (kJbRank + q) * kLanes;
+}
+
+// Jb += r k^T (one more factor pair; lane i holds r_i and k_i)
 template <int N>
 __device__ __forceinline__ void w_add_outer(WaveAdjLDS<N>& L, int n, double r, double k,
-                                            int lane) {
-    constexpr int P = WaveAdjLDS<N>::P;
-    L.v[lane] = k;
+                                            int lane, int& rank) {
+    jb_a(L, rank)[lane] = (lane < n) ? r : 0.0;
+    jb_b(L, rank)[lane] = (lane < n) ? k : 0.0;
+    ++rank;
     wave_sync();
-    if (lane < n && r != 0.0)
-        for (int j = 0; j < n; ++j) L.Jb[lane * P + j] += r * L.v[j];
-    wave_sync();
+}
+
+// sum_q a_q[i] b_q[j] for this lane's row i = lane and a column j (uniform)
+template <int N>
+__device__ __forceinline__ double jb_entry(WaveAdjLDS<N>& L, int rank, int i, int j) {
+    double v = 0.0;
+    for (int q = 0; q < rank; ++q) v += jb_a(L, q)[i] * jb_b(L, q)[j];
+    return v;
 }
 
 // Mixture quantities of the cell at y (adjoint_math.hpp mixture): wavefront
@@ -306,11 +339,13 @@ __device__ __forceinline__ void w_vjp_f(const DeviceMech& mr, WaveAdjLDS<N>& L, 
 
 // ybar += d/dy and th += d/dtheta of <Jb, J(y)> + <Fb, F(y)> for the fast
 // Jacobian J = L.f.J (pre-transform row quantities in L.raw) and F at y
-// (adjoint_math.hpp jact).  Jb is consumed.
+// (adjoint_math.hpp jact).  Jb (its factors, `rank` pairs) is consumed: the
+// reverse of each transform of gpuODESystem.H updates the factors (rows and
+// columns scaled, rank-one terms added) instead of the entries.
 template <int N>
 __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, double p,
                                        double y, double F, double Fb, double& ybar,
-                                       const adj::ThView& th, int lane) {
+                                       const adj::ThView& th, int lane, int rank) {
     constexpr int P = WaveAdjLDS<N>::P;
     const int nS = mr.nS, iT = nS;
     const double T = readlane(y, nS);
@@ -324,31 +359,36 @@ __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, d
     L.v[lane] = F;
     double Cpb = 0.0, Hab = 0.0, vb = 0.0, cbT = 0.0;
     adj::MixBar mb;
-    // ddT_dtdT (gpuODESystem.H:129-146)
+    // ddT_dtdT (gpuODESystem.H:129-146): Jb(i, T) += -g Ha_i / CpM, i species
     {
-        const double g = L.Jb[iT * P + iT];
+        const double g = jb_entry(L, rank, iT, iT);
         if (sp) {
             Fb += -g * x.Cp / CpM;
             Cpb += -g * F / CpM;
-            L.Jb[lane * P + iT] += -g * x.Ha / CpM;
             Hab += -g * L.f.J[lane * P + iT] / CpM;
         }
         if (lane == iT) Fb += -g * x.dCpMdT / CpM;
         mb.dCpMdT += -g * FT / CpM;
         mb.CpM += -g * L.f.J[iT * P + iT] / CpM;
+        jb_a(L, rank)[lane] = sp ? -g * x.Ha / CpM : 0.0;
+        jb_b(L, rank)[lane] = (lane == iT) ? 1.0 : 0.0;
+        ++rank;
     }
     wave_sync();
-    // ddT_dtdY (:108-126): lane j walks the columns i of its row
+    // ddT_dtdY (:108-126): Jb(i, j) += -Jb(T, j) Ha_i / CpM, i and j species
     {
+        // this lane's column of the temperature row
+        const double gi = sp ? jb_entry(L, rank, iT, lane) : 0.0;
+        jb_a(L, rank)[lane] = sp ? -x.Ha / CpM : 0.0;
+        jb_b(L, rank)[lane] = gi;
+        wave_sync();
         if (sp) {
-            for (int i = 0; i < nS; ++i) {
-                const double g = L.Jb[iT * P + i];
-                if (g == 0.0) continue;
-                L.Jb[lane * P + i] += -g * x.Ha / CpM;
-                Hab += -g * L.f.J[lane * P + i] / CpM;
-            }
+            const double* gT = jb_b(L, rank);
+            double h = 0.0;
+            for (int i = 0; i < nS; ++i) h += -gT[i] * L.f.J[lane * P + i] / CpM;
+            Hab += h;
         }
-        const double gi = sp ? L.Jb[iT * P + lane] : 0.0;
+        ++rank;
         double fbT = 0.0, cpm = 0.0;
         if (sp && gi != 0.0) {
             Cpb += -gi * FT / CpM;
@@ -363,38 +403,61 @@ __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, d
     // ddY_dcdY (:65-103).  Column sums first, from the entries before the
     // row transforms: vb_j (species columns) and the temperature column's
     // concentration adjoints.
+    double aF[kJbRank];
+#pragma unroll
+    for (int q = 0; q < kJbRank; ++q)
+        aF[q] = (q < rank) ? wave_sum(sp ? jb_a(L, q)[lane] * F : 0.0) : 0.0;
+    // this lane's row entry in the temperature column
+    const double gTi = sp ? jb_entry(L, rank, lane, iT) : 0.0;
+    const double gF = wave_sum(sp ? gTi * F : 0.0);
+    L.cb[lane] = gTi;
+    wave_sync();
     if (sp) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < kJbRank; ++q)
+            if (q < rank) s += jb_b(L, q)[lane] * aF[q];
+        vb = s * rho;
         // sum_i g_i (W_i/rho) t_ij with t the rows before the transform:
         // the transform made J_ij = (W_i/rho) t_ij (rho/W_j) + rho v_j F_i
-        double gJ = 0.0, gF = 0.0;
-        for (int i = 0; i < nS; ++i) {
-            vb += L.Jb[i * P + lane] * rho * L.v[i];
-            const double g = L.Jb[i * P + iT];
-            gJ += g * L.f.J[i * P + lane];
-            gF += g * L.v[i];
-        }
+        double gJ = 0.0;
+        for (int i = 0; i < nS; ++i) gJ += L.cb[i] * L.f.J[i * P + lane];
         cbT = -(gJ - (rho * x.vi) * gF) / (rho / mr.W[lane]) * av;
     }
     wave_sync();
     double rhoPart = 0.0, avPart = 0.0;
-    if (sp) {
-        const double WiR = mr.W[lane] / rho;
-        double* row = &L.Jb[lane * P];
-        for (int j = 0; j < nS; ++j) {
-            const double g = row[j];
-            rhoPart += g * L.vj[j] * F;
-            Fb += g * rho * L.vj[j];
-            row[j] = g * WiR * (rho / mr.W[j]);
+    {
+        // row sums with v_j, then the row and column scalings and the
+        // temperature column's rank-one term
+        double bv[kJbRank];
+#pragma unroll
+        for (int q = 0; q < kJbRank; ++q)
+            bv[q] = (q < rank) ? wave_sum(sp ? jb_b(L, q)[lane] * L.vj[lane] : 0.0) : 0.0;
+        if (sp) {
+            const double WiR = mr.W[lane] / rho;
+            double rs = 0.0;
+#pragma unroll
+            for (int q = 0; q < kJbRank; ++q)
+                if (q < rank) rs += jb_a(L, q)[lane] * bv[q];
+            rhoPart += rs * F;
+            Fb += rho * rs;
+            const double g = gTi;
+            const double sumc = L.raw[lane];
+            const double X = L.raw[kLanes + lane] - sumc * av;
+            avPart = g * (F - WiR * sumc);
+            rhoPart += -g * mr.W[lane] / (rho * rho) * X;
+            Fb += g * av;
+            for (int q = 0; q < rank; ++q) {
+                jb_a(L, q)[lane] *= WiR;
+                jb_b(L, q)[lane] *= rho / mr.W[lane];
+            }
+            jb_a(L, rank)[lane] = -g * WiR * av;
+            jb_b(L, rank)[lane] = L.f.c[lane];
+        } else {
+            jb_a(L, rank)[lane] = 0.0;
+            jb_b(L, rank)[lane] = 0.0;
         }
-        // temperature column
-        const double g = row[iT];
-        const double sumc = L.raw[lane];
-        const double X = L.raw[kLanes + lane] - sumc * av;
-        row[iT] = g * WiR;
-        for (int j = 0; j < nS; ++j) row[j] += -g * WiR * L.f.c[j] * av;
-        avPart = g * (F - WiR * sumc);
-        rhoPart += -g * mr.W[lane] / (rho * rho) * X;
-        Fb += g * av;
+        ++rank;
     }
     mb.rho += wave_sum(rhoPart);
     mb.alphav += wave_sum(avPart);
@@ -413,10 +476,12 @@ __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, d
         }
         mb.rho += wave_sum(part);
     }
+    wave_sync();
     L.cb[lane] = sp ? cbT : 0.0;
     wave_sync();
     // reactions (gpuReaction.H:547-702): adjoints of omega, dOmega/dT,
-    // kf dCf/dc_j, kr dCr/dc_j and D from the raw-entry adjoints
+    // kf dCf/dc_j, kr dCr/dc_j and D from the raw-entry adjoints; a
+    // reaction's combination of Jb rows is sum_q (sum_i nu_i a_q[i]) b_q
     double Tb = 0.0;
     const int nPass = (mr.nR + kLanes - 1) / kLanes;
 #pragma unroll 1
@@ -427,10 +492,22 @@ __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, d
         const int nl = mr.nl[r], nr = mr.nr[r];
         adj::RxnBarT<kTeamWidth> in;
         in.fast = true;
+        double al[kJbRank];
+#pragma unroll
+        for (int q = 0; q < kJbRank; ++q) {
+            double v = 0.0;
+            if (q < rank) {
+                const double* aq = jb_a(L, q);
+                for (int t = 0; t < kTeamWidth && t < nl; ++t) v -= mr.ls[o + t] * aq[mr.li[o + t]];
+                for (int t = 0; t < kTeamWidth && t < nr; ++t) v += mr.rs[o + t] * aq[mr.ri[o + t]];
+            }
+            al[q] = v;
+        }
         auto u = [&](int col) {
             double v = 0.0;
-            for (int q = 0; q < kTeamWidth && q < nl; ++q) v -= mr.ls[o + q] * L.Jb[mr.li[o + q] * P + col];
-            for (int q = 0; q < kTeamWidth && q < nr; ++q) v += mr.rs[o + q] * L.Jb[mr.ri[o + q] * P + col];
+#pragma unroll
+            for (int q = 0; q < kJbRank; ++q)
+                if (q < rank) v += al[q] * jb_b(L, q)[col];
             return v;
         };
         for (int q = 0; q < kTeamWidth && q < nl; ++q) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
@@ -474,11 +551,10 @@ __device__ __forceinline__ double w_step_reverse(const SolveArgs& A, const Devic
     (void)w_step<N, true>(A, L, p, y0, h, lane, S);
     double kb1 = t.b1 * yb, kb2 = t.b2 * yb, kb3 = t.b3 * yb, kb4 = t.b4 * yb;
     double Fb = 0.0, fb2 = 0.0, fb3 = 0.0;
-    for (int e = lane; e < N * WaveAdjLDS<N>::P; e += kLanes) L.Jb[e] = 0.0;
-    wave_sync();
+    int rank = 0; // factor pairs of Jb
     if (t.stages == 4) {
         const double rb = w_solveT<N>(S.col, n, S.ps, kb4, lane);
-        w_add_outer<N>(L, n, rb, S.k4, lane);
+        w_add_outer<N>(L, n, rb, S.k4, lane, rank);
         fb3 += rb;
         Fb += h * t.d4 * rb;
         kb1 += t.c41 / h * rb;
@@ -487,7 +563,7 @@ __device__ __forceinline__ double w_step_reverse(const SolveArgs& A, const Devic
     }
     if (t.stages >= 3) {
         const double rb = w_solveT<N>(S.col, n, S.ps, kb3, lane);
-        w_add_outer<N>(L, n, rb, S.k3, lane);
+        w_add_outer<N>(L, n, rb, S.k3, lane, rank);
         if (t.stages == 4) fb3 += rb;
         else fb2 += rb;
         Fb += h * t.d3 * rb;
@@ -504,7 +580,7 @@ __device__ __forceinline__ double w_step_reverse(const SolveArgs& A, const Devic
     }
     {
         const double rb = w_solveT<N>(S.col, n, S.ps, kb2, lane);
-        w_add_outer<N>(L, n, rb, S.k2, lane);
+        w_add_outer<N>(L, n, rb, S.k2, lane, rank);
         fb2 += rb;
         Fb += h * t.d2 * rb;
         kb1 += t.c21 / h * rb;
@@ -517,12 +593,12 @@ __device__ __forceinline__ double w_step_reverse(const SolveArgs& A, const Devic
         kb1 += t.a21 * ysb;
     }
     const double rb = w_solveT<N>(S.col, n, S.ps, kb1, lane);
-    w_add_outer<N>(L, n, rb, S.k1, lane);
+    w_add_outer<N>(L, n, rb, S.k1, lane, rank);
     Fb += h * t.d1 * rb;
     // within the reaction temperature limits f(y0) is the Jacobian call's F
     if (inRange) Fb += rb;
     else w_vjp_f<N>(mr, L, p, y0, rb, S.dydx0, yb, th, lane);
-    w_jact<N>(mr, L, p, y0, S.F, Fb, yb, th, lane);
+    w_jact<N>(mr, L, p, y0, S.F, Fb, yb, th, lane, rank);
     return yb;
 }
 

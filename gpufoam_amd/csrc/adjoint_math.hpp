@@ -301,10 +301,14 @@ using RxnBar = RxnBarT<kWidth>;
 // Reverse of reaction r at temperature T (clipped to [Tlow, Thigh] when
 // `clip`, as gpuReaction::omega does; the Jacobian uses T unclipped).
 // Adds to cbar, Tb and th[0..5].  W bounds the species per reaction side.
-template <int W, class CB>
+// With MSET the caller supplies the third-body concentration (Mset) and
+// receives its adjoint in *MbOut instead of the scatter eff_i * Mb into
+// cbar (a caller that sums the reactions of one efficiency set first).
+template <int W, class CB, bool MSET = false>
 GFC_HD __forceinline__ void reaction_reverse_w(const DeviceMech& m, int r, double T, bool clip,
                                             SV c, const RxnBarT<W>& in, CB cbar, double& Tb,
-                                            double* th) {
+                                            double* th, double Mset = 0.0,
+                                            double* MbOut = nullptr) {
     const double Te = clip ? fmin(fmax(T, m.Tlow), m.Thigh) : T;
     const double lnT = log(Te);
     const int t = m.type[r];
@@ -312,8 +316,8 @@ GFC_HD __forceinline__ void reaction_reverse_w(const DeviceMech& m, int r, doubl
     const long o = (long)r * kWidth;
     const bool rev = !m.irreversible[r];
     const double* eff = m.eff + (long)r * m.nS;
-    double M = 0.0;
-    if (t != GFC_RATE_ARRHENIUS)
+    double M = Mset;
+    if (!MSET && t != GFC_RATE_ARRHENIUS)
         for (int i = 0; i < m.nS; ++i) M += eff[i] * c[i];
     const double kA = arrhenius(kk, Te);
     const double qA = (kk[1] + kk[2] / Te) / Te;
@@ -406,7 +410,8 @@ GFC_HD __forceinline__ void reaction_reverse_w(const DeviceMech& m, int r, doubl
         arr_adjoint(kk, Te, lnT, kA, k0b, 0.0, TbE, th);
         arr_adjoint(ki, Te, lnT, kI, kIb, kqIb, TbE, th + 3);
     }
-    if (Mb != 0.0)
+    if (MSET) *MbOut = Mb;
+    else if (Mb != 0.0)
         for (int i = 0; i < m.nS; ++i) acc_add(cbar, i, eff[i] * Mb);
     if (!clip || (T >= m.Tlow && T <= m.Thigh)) Tb += TbE;
 }

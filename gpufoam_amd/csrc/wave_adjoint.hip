@@ -182,10 +182,11 @@ __device__ __forceinline__ double w_solveT(const double (&col)[N], int n, int ps
     double w = 0.0;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        if (j >= n) break;
-        const double wj = readlane(r, j) / readlane(col[j], j);
-        if (lane == j) w = wj;
-        if (lane > j && lane < n) r -= col[j] * wj;
+        if (j < n) {
+            const double wj = readlane(r, j) / readlane(col[j], j);
+            if (lane == j) w = wj;
+            if (lane > j && lane < n) r -= col[j] * wj;
+        }
     }
     // L^T v = w (unit diagonal)
     double v = 0.0;
@@ -225,6 +226,54 @@ __device__ __forceinline__ double jb_entry(WaveAdjLDS<N>& L, int rank, int i, in
     double v = 0.0;
     for (int q = 0; q < rank; ++q) v += jb_a(L, q)[i] * jb_b(L, q)[j];
     return v;
+}
+
+// Third-body terms of the reaction sweeps, by efficiency set: the sweeps
+// take each set's M from L.f.M (third_body) and sum the reactions' M
+// adjoints per set, then scatter them over the species once.  The per-set
+// sums live in the specie Gibbs arrays, which only the forward evaluations
+// use; the Jacobian adjoint's per-set products eff_s . b_q (sum_j eff_sj
+// b_q[j], rank * nSets <= 128 of them) in the enthalpy arrays.
+static_assert(kTeamMaxSets <= 2 * kLanes, "per-set sums in gL and gH");
+template <int N>
+__device__ __forceinline__ double* mb_set(WaveAdjLDS<N>& L, int s) {
+    return s < kLanes ? &L.f.gL[s] : &L.f.gH[s - kLanes];
+}
+template <int N>
+__device__ __forceinline__ double* eb_slot(WaveAdjLDS<N>& L, int t) {
+    return t < kLanes ? &L.f.hL[t] : &L.f.hH[t - kLanes];
+}
+
+// M of every set at the c in L.f.c, and the per-set M adjoints cleared.
+template <int N>
+__device__ __forceinline__ void w_sets_begin(const DeviceMech& mr, WaveAdjLDS<N>& L, int lane) {
+    third_body<N>(mr, L.f, lane);
+    for (int s = lane; s < mr.nSets; s += kLanes) *mb_set(L, s) = 0.0;
+    wave_sync();
+}
+
+// This specie's part of the per-set M adjoints: sum_s eff_s,i Mb_s.
+template <int N>
+__device__ __forceinline__ double w_sets_scatter(const DeviceMech& mr, WaveAdjLDS<N>& L,
+                                                 int lane) {
+    double v = 0.0;
+    if (lane < mr.nS)
+        for (int s = 0; s < mr.nSets; ++s) v += mr.effSets[s * kLanes + lane] * *mb_set(L, s);
+    return v;
+}
+
+// One reaction's reverse with its set's M, its M adjoint summed per set.
+template <int N>
+__device__ __forceinline__ void w_reaction_reverse(const DeviceMech& mr, WaveAdjLDS<N>& L,
+                                                   int r, double T, bool clip,
+                                                   const adj::RxnBarT<kTeamWidth>& in,
+                                                   double& Tb, double* t6) {
+    const int set = mr.effSet[r];
+    double Mb = 0.0;
+    adj::reaction_reverse_w<kTeamWidth, adj::WaveAcc, true>(
+        mr, r, T, clip, lane::SV{L.f.c, 1}, in, adj::WaveAcc{L.cb}, Tb, t6,
+        set >= 0 ? L.f.M[set] : 0.0, &Mb);
+    if (set >= 0 && Mb != 0.0) atomicAdd(mb_set(L, set), Mb);
 }
 
 // Mixture quantities of the cell at y (adjoint_math.hpp mixture): wavefront
@@ -314,6 +363,7 @@ __device__ __forceinline__ void w_vjp_f(const DeviceMech& mr, WaveAdjLDS<N>& L, 
     L.cb[lane] = 0.0;
     L.f.c[lane] = (lane < nS) ? x.c : 0.0;
     wave_sync();
+    w_sets_begin<N>(mr, L, lane);
     double Tb = 0.0;
     const int nPass = (mr.nR + kLanes - 1) / kLanes;
 #pragma unroll 1
@@ -326,13 +376,12 @@ __device__ __forceinline__ void w_vjp_f(const DeviceMech& mr, WaveAdjLDS<N>& L, 
         for (int q = 0; q < kTeamWidth && q < mr.nr[r]; ++q) in.w += mr.rs[o + q] * L.dNb[mr.ri[o + q]];
         if (in.w == 0.0) continue;
         double t6[adj::kParams] = {0, 0, 0, 0, 0, 0};
-        adj::reaction_reverse_w<kTeamWidth>(mr, r, T, true, lane::SV{L.f.c, 1}, in,
-                                            adj::WaveAcc{L.cb}, Tb, t6);
+        w_reaction_reverse<N>(mr, L, r, T, true, in, Tb, t6);
         th.add(mr.rxnRef[r], t6);
     }
     Tb = wave_sum(Tb);
     wave_sync();
-    const double cb = (lane < nS) ? L.cb[lane] : 0.0;
+    const double cb = (lane < nS) ? L.cb[lane] + w_sets_scatter<N>(mr, L, lane) : 0.0;
     w_mixture_reverse(mr, p, y, T, x, cb, 0.0, Hab, 0.0, false, mb, Tb, ybar, lane);
     wave_sync();
 }
@@ -479,6 +528,22 @@ __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, d
     wave_sync();
     L.cb[lane] = sp ? cbT : 0.0;
     wave_sync();
+    w_sets_begin<N>(mr, L, lane);
+    // eff_s . b_q for the third-body reactions' D = sum_j eff_j u(j)
+    const bool setProducts = mr.nSets * kJbRank <= 2 * kLanes;
+    if (setProducts) {
+        for (int t = lane; t < mr.nSets * kJbRank; t += kLanes) {
+            const int s = t / kJbRank, q = t % kJbRank;
+            double v = 0.0;
+            if (q < rank) {
+                const double* e = mr.effSets + s * kLanes;
+                const double* b = jb_b(L, q);
+                for (int j = 0; j < nS; ++j) v += e[j] * b[j];
+            }
+            *eb_slot(L, t) = v;
+        }
+        wave_sync();
+    }
     // reactions (gpuReaction.H:547-702): adjoints of omega, dOmega/dT,
     // kf dCf/dc_j, kr dCr/dc_j and D from the raw-entry adjoints; a
     // reaction's combination of Jb rows is sum_q (sum_i nu_i a_q[i]) b_q
@@ -516,20 +581,26 @@ __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, d
         for (int j = 0; j < kTeamWidth && j < nr; ++j) in.dCr[j] = -u(mr.ri[o + j]);
         in.dwdT = u(iT);
         if (mr.type[r] == GFC_RATE_THIRD_BODY) {
-            const double* eff = mr.eff + (long)r * nS;
             double D = 0.0;
-            for (int j = 0; j < nS; ++j)
-                if (eff[j] != 0.0) D += eff[j] * u(j);
+            if (setProducts) {
+                const int set = mr.effSet[r];
+#pragma unroll
+                for (int q = 0; q < kJbRank; ++q)
+                    if (q < rank) D += al[q] * *eb_slot(L, set * kJbRank + q);
+            } else {
+                const double* eff = mr.eff + (long)r * nS;
+                for (int j = 0; j < nS; ++j)
+                    if (eff[j] != 0.0) D += eff[j] * u(j);
+            }
             in.D = D;
         }
         double t6[adj::kParams] = {0, 0, 0, 0, 0, 0};
-        adj::reaction_reverse_w<kTeamWidth>(mr, r, T, false, lane::SV{L.f.c, 1}, in,
-                                            adj::WaveAcc{L.cb}, Tb, t6);
+        w_reaction_reverse<N>(mr, L, r, T, false, in, Tb, t6);
         th.add(mr.rxnRef[r], t6);
     }
     Tb = wave_sum(Tb);
     wave_sync();
-    const double cb = sp ? L.cb[lane] : 0.0;
+    const double cb = sp ? L.cb[lane] + w_sets_scatter<N>(mr, L, lane) : 0.0;
     w_mixture_reverse(mr, p, y, T, x, cb, Cpb, Hab, vb, true, mb, Tb, ybar, lane);
     wave_sync();
 }

@@ -47,7 +47,8 @@ __global__ void team_record_kernel(SolveArgs A, int* queue, t16::T16Record R);
 template <int N>
 __global__ void wave_adjoint_reverse_kernel(SolveArgs A, t16::T16Record R, double* seg,
                                             long segPer, const double* Ybar, long ldb,
-                                            double* Y0bar, double* thBar, int* queue);
+                                            double* Y0bar, double* thBar, double* pacc,
+                                            long paccSlot, long paccPer, int* queue);
 
 namespace {
 
@@ -294,6 +295,11 @@ static t16::T16Record record_of(const AdjState* st, const AdjLayout& W) {
 static int wave_adjoint_rows(const gfc_evaluator* ev) {
     if (ev->cellsPerWave != 0 && ev->cellsPerWave != 1 && ev->cellsPerWave != 2) return 0;
     if (ev->nS + 1 <= 16) return 0;
+    // the reverse keeps a cell's parameter adjoints in the cell's share of
+    // the one-lane scratch, which it does not otherwise use
+    if ((long)adj::kParams * 64 * ev->dmTeam.nPass >
+        lane::lane_scratch_doubles(ev->nS) + adj::adj_scratch_doubles(ev->nS))
+        return 0;
     return wave_kernel_rows(ev);
 }
 
@@ -510,7 +516,8 @@ int gfc_adjoint_backward(gfc_evaluator* ev, const double* Ybar, int ldb, double*
         hipLaunchKernelGGL(adj_reset_queue_kernel, dim3(1), dim3(1), 0, s, st->queue);
         hipLaunchKernelGGL(k, dim3(resident_grid(st, ev->device, k, st->nCells, 1)), dim3(64), 0,
                            s, At, record_of(st, W), seg, W.perScratch, Ybar, (long)ldb, Y0bar,
-                           paramBar, st->queue);
+                           paramBar, st->scratch, W.perScratch * lane::kSlotStride,
+                           W.laneD + W.adjD, st->queue);
     } else if (st->teamN) {
         SolveArgs At = A;
         At.m = ev->dmTeam;

@@ -50,6 +50,21 @@ __device__ __forceinline__ void acc_add(WaveAcc v, int i, double x) { atomicAdd(
 
 namespace {
 
+// Rate-parameter adjoints of one cell while its steps are reversed: a
+// contiguous per-cell block in the order the sweeps visit the reactions,
+// acc[(pass*kParams + q)*64 + lane] for reaction pass*64 + lane, so each
+// sweep's additions are coalesced; written to thBar in reference order
+// when the cell ends.
+struct WTh {
+    double* acc; // nullptr: no parameter adjoints
+    __device__ __forceinline__ void add(int pass, int lane, const double* v) const {
+        if (!acc) return;
+#pragma unroll
+        for (int q = 0; q < adj::kParams; ++q)
+            if (v[q] != 0.0) acc[(pass * adj::kParams + q) * kLanes + lane] += v[q];
+    }
+};
+
 using t16::kRecStride;
 using t16::rec_slot;
 using t16::T16Record;
@@ -344,7 +359,7 @@ __device__ __forceinline__ void w_mixture_reverse(const DeviceMech& m, double p,
 template <int N>
 __device__ __forceinline__ void w_vjp_f(const DeviceMech& mr, WaveAdjLDS<N>& L, double p,
                                         double y, double fbar, double fy, double& ybar,
-                                        const adj::ThView& th, int lane) {
+                                        const WTh& th, int lane) {
     const int nS = mr.nS;
     const double T = readlane(y, nS);
     const WMix x = w_mixture(mr, p, y, T, lane);
@@ -377,7 +392,7 @@ __device__ __forceinline__ void w_vjp_f(const DeviceMech& mr, WaveAdjLDS<N>& L, 
         if (in.w == 0.0) continue;
         double t6[adj::kParams] = {0, 0, 0, 0, 0, 0};
         w_reaction_reverse<N>(mr, L, r, T, true, in, Tb, t6);
-        th.add(mr.rxnRef[r], t6);
+        th.add(pass, lane, t6);
     }
     Tb = wave_sum(Tb);
     wave_sync();
@@ -394,7 +409,7 @@ __device__ __forceinline__ void w_vjp_f(const DeviceMech& mr, WaveAdjLDS<N>& L, 
 template <int N>
 __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, double p,
                                        double y, double F, double Fb, double& ybar,
-                                       const adj::ThView& th, int lane, int rank) {
+                                       const WTh& th, int lane, int rank) {
     constexpr int P = WaveAdjLDS<N>::P;
     const int nS = mr.nS, iT = nS;
     const double T = readlane(y, nS);
@@ -596,7 +611,7 @@ __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, d
         }
         double t6[adj::kParams] = {0, 0, 0, 0, 0, 0};
         w_reaction_reverse<N>(mr, L, r, T, false, in, Tb, t6);
-        th.add(mr.rxnRef[r], t6);
+        th.add(pass, lane, t6);
     }
     Tb = wave_sum(Tb);
     wave_sync();
@@ -611,7 +626,7 @@ __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, d
 template <int N>
 __device__ __forceinline__ double w_step_reverse(const SolveArgs& A, const DeviceMech& mr,
                                                  WaveAdjLDS<N>& L, double p, double y0,
-                                                 double h, double yb, const adj::ThView& th,
+                                                 double h, double yb, const WTh& th,
                                                  int lane) {
     const DeviceMech& m = A.m;
     const int nS = m.nS, n = nS + 1;
@@ -678,12 +693,14 @@ __device__ __forceinline__ double w_step_reverse(const SolveArgs& A, const Devic
 // Reverse sweep, one wavefront per cell (adjoint.hip adjoint_reverse_kernel).
 // A.m: the wavefront solve's tables.  seg: the K-state segment buffers
 // (segPer doubles per cell, wave-blocked).  Ybar/Y0bar: [n][ldb]; thBar:
-// [6 nR][ldb], reactions in reference order.
+// [6 nR][ldb], reactions in reference order.  pacc: the parameter
+// accumulators, cell c at pacc + (c/64)*paccSlot + (c%64)*paccPer, each
+// kParams*64*nPass doubles long.
 template <int N>
 __global__ void __launch_bounds__(64)
 wave_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer,
                             const double* Ybar, long ldb, double* Y0bar, double* thBar,
-                            int* queue) {
+                            double* pacc, long paccSlot, long paccPer, int* queue) {
     __shared__ WaveAdjLDS<N> L;
     const int lane = threadIdx.x;
     const DeviceMech& mr = A.m;
@@ -697,9 +714,11 @@ wave_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer,
         const double p = y0c[(long)n * kRecStride];
         const double T0 = y0c[(long)nS * kRecStride];
         double yb = (lane < n) ? Ybar[lane * ldb + cell] : 0.0;
-        const adj::ThView th{thBar ? thBar + cell : nullptr, ldb};
-        if (th.p)
-            for (long q = lane; q < (long)adj::kParams * mr.nR; q += kLanes) th.p[q * ldb] = 0.0;
+        const int nAccP = adj::kParams * kLanes * mr.nPass;
+        const WTh th{thBar ? pacc + (cell / kLanes) * paccSlot + (cell % kLanes) * paccPer
+                           : nullptr};
+        if (th.acc)
+            for (int q = lane; q < nAccP; q += kLanes) th.acc[q] = 0.0;
         if (T0 > A.Treact && A.deltaT > kSmall) {
             const double* yE = rec_slot(R.yEnd, cell, n);
             if (lane < nS && yE[lane * kRecStride] < 0.0) yb = 0.0;
@@ -725,18 +744,28 @@ wave_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer,
             }
         }
         if (lane < n) Y0bar[lane * ldb + cell] = yb;
+        if (th.acc)
+            for (int pass = 0; pass < mr.nPass; ++pass) {
+                const int r = pass * kLanes + lane;
+                if (r >= mr.nR) break;
+                const long ref = mr.rxnRef[r];
+#pragma unroll
+                for (int q = 0; q < adj::kParams; ++q)
+                    thBar[(adj::kParams * ref + q) * ldb + cell] =
+                        th.acc[(pass * adj::kParams + q) * kLanes + lane];
+            }
         wave_sync();
     }
 }
 
 template __global__ void wave_adjoint_reverse_kernel<32>(SolveArgs, T16Record, double*, long,
                                                          const double*, long, double*, double*,
-                                                         int*);
+                                                         double*, long, long, int*);
 template __global__ void wave_adjoint_reverse_kernel<54>(SolveArgs, T16Record, double*, long,
                                                          const double*, long, double*, double*,
-                                                         int*);
+                                                         double*, long, long, int*);
 template __global__ void wave_adjoint_reverse_kernel<64>(SolveArgs, T16Record, double*, long,
                                                          const double*, long, double*, double*,
-                                                         int*);
+                                                         double*, long, long, int*);
 
 } // namespace gfc

@@ -168,16 +168,23 @@ GFC_HD inline void mixture_reverse(const DeviceMech& m, double p, SV y, const Mi
 
 // Adjoint of an Arrhenius expression k = A T^b exp(-Ta/T) with respect to
 // T and (ln|A|, b, Ta), given the adjoints of its value k and of its
-// temperature derivative k q, q = (b + Ta/T)/T.
-GFC_HD __forceinline__ void arr_adjoint(const double* kk, double T, double lnT, double k,
-                                        double kb, double kqb, double& Tb, double* th) {
+// temperature derivative k q, q = (b + Ta/T)/T.  Returned by value (not
+// added through a pointer), so that a caller's per-reaction accumulators
+// stay in registers.
+struct ArrBar {
+    double T, lnA, beta, Ta;
+};
+GFC_HD __forceinline__ ArrBar arr_adjoint(const double* kk, double T, double lnT, double k,
+                                          double kb, double kqb) {
     const double be = kk[1], Ta = kk[2];
     const double q = (be + Ta / T) / T;
     const double dq = -(be + 2.0 * Ta / T) / (T * T);
-    Tb += kb * k * q + kqb * (k * q * q + k * dq);
-    th[0] += kb * k + kqb * k * q;
-    th[1] += kb * k * lnT + kqb * (k * lnT * q + k / T);
-    th[2] += -kb * k / T + kqb * (k / (T * T) - k * q / T);
+    ArrBar a;
+    a.T = kb * k * q + kqb * (k * q * q + k * dq);
+    a.lnA = kb * k + kqb * k * q;
+    a.beta = kb * k * lnT + kqb * (k * lnT * q + k / T);
+    a.Ta = -kb * k / T + kqb * (k / (T * T) - k * q / T);
+    return a;
 }
 
 // Troe blending function F(T, Pr) (gpuTroeFallOffFunction.H:27-47) and its
@@ -239,7 +246,9 @@ GFC_HD __forceinline__ double specie_pow_m2(double x, int ei, double er) {
 GFC_HD __forceinline__ void acc_add(SV v, int i, double x) { v[i] += x; }
 
 // One side of a reaction: C = prod c^e, dC[j] = dC/dc_j (product without j).
-// W bounds the number of species on a side.
+// W bounds the number of species on a side.  The loops run to W with the
+// entries beyond n masked, so that they unroll and the arrays stay in
+// registers.
 template <int W>
 struct SideT {
     int n;
@@ -249,36 +258,51 @@ struct SideT {
     double pw[W], d1[W], dC[W];
     double C;
     GFC_HD void eval(SV c) {
-        for (int q = 0; q < W && q < n; ++q) {
-            pw[q] = specie_pow(c[idx[q]], ei[q], er[q]);
-            d1[q] = er[q] * specie_pow_m1(c[idx[q]], ei[q], er[q]);
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            pw[q] = 1.0;
+            d1[q] = 0.0;
+            if (q < n) {
+                pw[q] = specie_pow(c[idx[q]], ei[q], er[q]);
+                d1[q] = er[q] * specie_pow_m1(c[idx[q]], ei[q], er[q]);
+            }
         }
         C = 1.0;
-        for (int q = 0; q < W && q < n; ++q) C *= pw[q];
-        for (int j = 0; j < W && j < n; ++j) {
+#pragma unroll
+        for (int q = 0; q < W; ++q)
+            if (q < n) C *= pw[q];
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
             double v = 1;
-            for (int i = 0; i < W && i < n; ++i) v *= (i == j) ? d1[i] : pw[i];
+#pragma unroll
+            for (int i = 0; i < W; ++i)
+                if (i < n) v *= (i == j) ? d1[i] : pw[i];
             dC[j] = v;
         }
     }
     // cbar[idx[l]] += Cb dC/dc_l + sum_j dCb[j] d2C/dc_j dc_l
     template <class CB>
     GFC_HD void reverse(SV c, double Cb, const double* dCb, CB cbar) const {
-        for (int l = 0; l < W && l < n; ++l) {
+#pragma unroll
+        for (int l = 0; l < W; ++l) {
+            if (l >= n) continue;
             double acc = Cb * dC[l];
-            for (int j = 0; j < W && j < n; ++j) {
-                if (dCb[j] == 0.0) continue;
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                if (j >= n || dCb[j] == 0.0) continue;
                 double h;
                 if (j == l) {
                     const double coef = er[j] * (er[j] - 1.0);
                     if (coef == 0.0) continue;
                     h = coef * specie_pow_m2(c[idx[j]], ei[j], er[j]);
-                    for (int q = 0; q < W && q < n; ++q)
-                        if (q != j) h *= pw[q];
+#pragma unroll
+                    for (int q = 0; q < W; ++q)
+                        if (q < n && q != j) h *= pw[q];
                 } else {
                     h = d1[j] * d1[l];
-                    for (int q = 0; q < W && q < n; ++q)
-                        if (q != j && q != l) h *= pw[q];
+#pragma unroll
+                    for (int q = 0; q < W; ++q)
+                        if (q < n && q != j && q != l) h *= pw[q];
                 }
                 acc += dCb[j] * h;
             }
@@ -363,15 +387,19 @@ GFC_HD __forceinline__ void reaction_reverse_w(const DeviceMech& m, int r, doubl
         Crb += -in.dwdT * dkrdT;
         dkfdTb = in.dwdT * Cf;
         dkrdTb = -in.dwdT * Cr;
-        for (int j = 0; j < W && j < L.n; ++j) {
-            kfb += in.dCf[j] * L.dC[j];
-            dCLb[j] = in.dCf[j] * kf;
-        }
-        if (rev)
-            for (int j = 0; j < W && j < R.n; ++j) {
-                krb += in.dCr[j] * R.dC[j];
-                dCRb[j] = in.dCr[j] * kr;
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+            if (j < L.n) {
+                kfb += in.dCf[j] * L.dC[j];
+                dCLb[j] = in.dCf[j] * kf;
             }
+        if (rev)
+#pragma unroll
+            for (int j = 0; j < W; ++j)
+                if (j < R.n) {
+                    krb += in.dCr[j] * R.dC[j];
+                    dCRb[j] = in.dCr[j] * kr;
+                }
         if (t == GFC_RATE_THIRD_BODY) {
             Cfb += in.D * kA;
             Crb += -in.D * kA / K.Kc;
@@ -392,11 +420,12 @@ GFC_HD __forceinline__ void reaction_reverse_w(const DeviceMech& m, int r, doubl
     L.reverse(c, Cfb, dCLb, cbar);
     if (rev) R.reverse(c, Crb, dCRb, cbar);
     double Mb = 0.0;
+    ArrBar a0{0.0, 0.0, 0.0, 0.0}, a1{0.0, 0.0, 0.0, 0.0};
     if (t == GFC_RATE_ARRHENIUS) {
-        arr_adjoint(kk, Te, lnT, kA, kfb, dkfdTb, TbE, th);
+        a0 = arr_adjoint(kk, Te, lnT, kA, kfb, dkfdTb);
     } else if (t == GFC_RATE_THIRD_BODY) {
         Mb = kfb * kA + dkfdTb * (kA * qA);
-        arr_adjoint(kk, Te, lnT, kA, kfb * M + kAb, dkfdTb * M, TbE, th);
+        a0 = arr_adjoint(kk, Te, lnT, kA, kfb * M + kAb, dkfdTb * M);
     } else {
         double kIb = kfb * temp;
         const double tempb = kfb * kI + dkfdTb * (kI * qI);
@@ -407,9 +436,17 @@ GFC_HD __forceinline__ void reaction_reverse_w(const DeviceMech& m, int r, doubl
         const double k0b = Prb * M / kI;
         Mb = Prb * kA / kI;
         kIb += -Prb * Pr / kI;
-        arr_adjoint(kk, Te, lnT, kA, k0b, 0.0, TbE, th);
-        arr_adjoint(ki, Te, lnT, kI, kIb, kqIb, TbE, th + 3);
+        a0 = arr_adjoint(kk, Te, lnT, kA, k0b, 0.0);
+        a1 = arr_adjoint(ki, Te, lnT, kI, kIb, kqIb);
     }
+    TbE += a0.T;
+    TbE += a1.T;
+    th[0] += a0.lnA;
+    th[1] += a0.beta;
+    th[2] += a0.Ta;
+    th[3] += a1.lnA;
+    th[4] += a1.beta;
+    th[5] += a1.Ta;
     if (MSET) *MbOut = Mb;
     else if (Mb != 0.0)
         for (int i = 0; i < m.nS; ++i) acc_add(cbar, i, eff[i] * Mb);

@@ -387,8 +387,13 @@ __device__ __forceinline__ void w_vjp_f(const DeviceMech& mr, WaveAdjLDS<N>& L, 
         if (r >= mr.nR) continue;
         const long o = (long)r * kWidth;
         adj::RxnBarT<kTeamWidth> in;
-        for (int q = 0; q < kTeamWidth && q < mr.nl[r]; ++q) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
-        for (int q = 0; q < kTeamWidth && q < mr.nr[r]; ++q) in.w += mr.rs[o + q] * L.dNb[mr.ri[o + q]];
+        const int nl = mr.nl[r], nr = mr.nr[r];
+#pragma unroll
+        for (int q = 0; q < kTeamWidth; ++q)
+            if (q < nl) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
+#pragma unroll
+        for (int q = 0; q < kTeamWidth; ++q)
+            if (q < nr) in.w += mr.rs[o + q] * L.dNb[mr.ri[o + q]];
         if (in.w == 0.0) continue;
         double t6[adj::kParams] = {0, 0, 0, 0, 0, 0};
         w_reaction_reverse<N>(mr, L, r, T, true, in, Tb, t6);
@@ -590,10 +595,18 @@ __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, d
                 if (q < rank) v += al[q] * jb_b(L, q)[col];
             return v;
         };
-        for (int q = 0; q < kTeamWidth && q < nl; ++q) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
-        for (int q = 0; q < kTeamWidth && q < nr; ++q) in.w += mr.rs[o + q] * L.dNb[mr.ri[o + q]];
-        for (int j = 0; j < kTeamWidth && j < nl; ++j) in.dCf[j] = u(mr.li[o + j]);
-        for (int j = 0; j < kTeamWidth && j < nr; ++j) in.dCr[j] = -u(mr.ri[o + j]);
+#pragma unroll
+        for (int q = 0; q < kTeamWidth; ++q)
+            if (q < nl) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
+#pragma unroll
+        for (int q = 0; q < kTeamWidth; ++q)
+            if (q < nr) in.w += mr.rs[o + q] * L.dNb[mr.ri[o + q]];
+#pragma unroll
+        for (int j = 0; j < kTeamWidth; ++j)
+            if (j < nl) in.dCf[j] = u(mr.li[o + j]);
+#pragma unroll
+        for (int j = 0; j < kTeamWidth; ++j)
+            if (j < nr) in.dCr[j] = -u(mr.ri[o + j]);
         in.dwdT = u(iT);
         if (mr.type[r] == GFC_RATE_THIRD_BODY) {
             double D = 0.0;

@@ -39,7 +39,8 @@ __global__ void team16_record_kernel(SolveArgs A, int* queue, t16::T16Record R);
 template <int N>
 __global__ void team16_adjoint_reverse_kernel(SolveArgs A, t16::T16Record R, double* seg,
                                               long segPer, const double* Ybar, long ldb,
-                                              double* Y0bar, double* thBar, int* queue);
+                                              double* Y0bar, double* thBar, double* pacc,
+                                              long paccSlot, long paccPer, int* queue);
 // One wavefront per cell (team_kernel.hip, wave_adjoint.hip): used for
 // 16 < nSpecie+1 <= 64 unless one lane per cell is asked for.
 template <int N>
@@ -458,7 +459,11 @@ int gfc_adjoint_forward(gfc_evaluator* ev, double deltaT, int nCells, int ldY,
     A.dtChem = deltaTChem;
     A.stats = cellStats;
     int N = 0;
-    st->teamN = team16_selected(ev, nCells, N) ? N : 0;
+    // the reverse keeps a cell's parameter adjoints in the cell's share of
+    // the one-lane scratch, which the team form does not otherwise use
+    const bool teamAccFits = (long)adj::kParams * 16 * ((ev->nR + 15) / 16) <=
+                             lane::lane_scratch_doubles(ev->nS) + adj::adj_scratch_doubles(ev->nS);
+    st->teamN = (teamAccFits && team16_selected(ev, nCells, N)) ? N : 0;
     st->waveN = st->teamN ? 0 : wave_adjoint_rows(ev);
     if (st->waveN) {
         SolveArgs At = A;
@@ -528,7 +533,8 @@ int gfc_adjoint_backward(gfc_evaluator* ev, const double* Ybar, int ldb, double*
         hipLaunchKernelGGL(adj_reset_queue_kernel, dim3(1), dim3(1), 0, s, st->queue);
         hipLaunchKernelGGL(k, dim3(team16_grid(st, ev->device, k, st->nCells)), dim3(64), 0, s, At,
                            record_of(st, W), seg, W.perScratch, Ybar, (long)ldb, Y0bar,
-                           paramBar, st->queue);
+                           paramBar, st->scratch, W.perScratch * lane::kSlotStride,
+                           W.laneD + W.adjD, st->queue);
     } else {
         const int blocks = (st->nCells + 63) / 64;
         hipLaunchKernelGGL(adjoint_reverse_kernel, dim3(blocks), dim3(64), 0, s, A, W,

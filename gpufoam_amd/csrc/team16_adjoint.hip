@@ -60,6 +60,53 @@ struct T16AdjLDS {
     __device__ __forceinline__ double* Jb() { return LU() + N * P; }
 };
 
+// Rate-parameter adjoints of one cell while its steps are reversed: a
+// contiguous per-cell block in the order the sweeps visit the reactions,
+// acc[(pass*kParams + q)*16 + mem] for reaction pass*16 + mem, so a team's
+// additions are coalesced; written to thBar in reference order when the
+// cell ends.
+struct T16Th {
+    double* acc; // nullptr: no parameter adjoints
+    __device__ __forceinline__ void add(int pass, int mem, const double* v) const {
+        if (!acc) return;
+#pragma unroll
+        for (int q = 0; q < adj::kParams; ++q)
+            if (v[q] != 0.0) acc[(pass * adj::kParams + q) * kTW + mem] += v[q];
+    }
+};
+
+// Third-body terms of the reaction sweeps, by efficiency set: M of each set
+// from third_body16, the reactions' M adjoints summed per set in f.gL (the
+// specie Gibbs array, used only by the forward evaluations) and scattered
+// over the species once.
+static_assert(kSets16 <= kTW, "per-set sums in gL");
+template <int N>
+__device__ __forceinline__ void t16_sets_begin(const DeviceMech& mr, T16AdjLDS<N>& L, int mem) {
+    third_body16<N>(mr, L.f, mem);
+    if (mem < mr.nSets) L.f.gL[mem] = 0.0;
+    wave_sync();
+}
+template <int N>
+__device__ __forceinline__ double t16_sets_scatter(const DeviceMech& mr, T16AdjLDS<N>& L,
+                                                   int mem) {
+    double v = 0.0;
+    if (mem < mr.nS)
+        for (int s = 0; s < mr.nSets; ++s) v += mr.effSets[s * kLanes + mem] * L.f.gL[s];
+    return v;
+}
+template <int N>
+__device__ __forceinline__ void t16_reaction_reverse(const DeviceMech& mr, T16AdjLDS<N>& L,
+                                                     int r, double T, bool clip,
+                                                     const adj::RxnBarT<kTeamWidth>& in,
+                                                     double& Tb, double* t6) {
+    const int set = mr.effSet[r];
+    double Mb = 0.0;
+    adj::reaction_reverse_w<kTeamWidth, adj::LdsAcc, true>(
+        mr, r, T, clip, lane::SV{L.f.c, 1}, in, adj::LdsAcc{L.cb}, Tb, t6,
+        set >= 0 ? L.f.M[set] : 0.0, &Mb);
+    if (set >= 0 && Mb != 0.0) atomicAdd(&L.f.gL[set], Mb);
+}
+
 // One step's recomputed quantities, this member's components.
 struct T16Stages {
     double F, dydx0, k1, k2, k3, k4, f2, f3;
@@ -241,7 +288,7 @@ __device__ __forceinline__ void t16_mixture_reverse(const DeviceMech& m, double 
 template <int N>
 __device__ __forceinline__ void t16_vjp_f(const DeviceMech& mr, T16AdjLDS<N>& L, double p,
                                           double y, double fbar, double fy, double& ybar,
-                                          const adj::ThView& th, int lane, int mem) {
+                                          const T16Th& th, int lane, int mem) {
     const int nS = mr.nS;
     const double T = team_bcast(y, lane, nS);
     const T16Mix x = t16_mixture(mr, p, y, T, lane, mem);
@@ -260,6 +307,7 @@ __device__ __forceinline__ void t16_vjp_f(const DeviceMech& mr, T16AdjLDS<N>& L,
     L.cb[mem] = 0.0;
     L.f.c[mem] = (mem < nS) ? x.c : 0.0;
     wave_sync();
+    t16_sets_begin<N>(mr, L, mem);
     double Tb = 0.0;
     const int nPass = (mr.nR + kTW - 1) / kTW;
 #pragma unroll 1
@@ -268,17 +316,21 @@ __device__ __forceinline__ void t16_vjp_f(const DeviceMech& mr, T16AdjLDS<N>& L,
         if (r >= mr.nR) continue;
         const long o = (long)r * kWidth;
         adj::RxnBarT<kTeamWidth> in;
-        for (int q = 0; q < kTeamWidth && q < mr.nl[r]; ++q) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
-        for (int q = 0; q < kTeamWidth && q < mr.nr[r]; ++q) in.w += mr.rs[o + q] * L.dNb[mr.ri[o + q]];
+        const int nl = mr.nl[r], nr = mr.nr[r];
+#pragma unroll
+        for (int q = 0; q < kTeamWidth; ++q)
+            if (q < nl) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
+#pragma unroll
+        for (int q = 0; q < kTeamWidth; ++q)
+            if (q < nr) in.w += mr.rs[o + q] * L.dNb[mr.ri[o + q]];
         if (in.w == 0.0) continue;
         double t6[adj::kParams] = {0, 0, 0, 0, 0, 0};
-        adj::reaction_reverse_w<kTeamWidth>(mr, r, T, true, lane::SV{L.f.c, 1}, in,
-                                            adj::LdsAcc{L.cb}, Tb, t6);
-        th.add(mr.rxnRef[r], t6);
+        t16_reaction_reverse<N>(mr, L, r, T, true, in, Tb, t6);
+        th.add(pass, mem, t6);
     }
     Tb = team_sum(Tb, lane);
     wave_sync();
-    const double cb = (mem < nS) ? L.cb[mem] : 0.0;
+    const double cb = (mem < nS) ? L.cb[mem] + t16_sets_scatter<N>(mr, L, mem) : 0.0;
     t16_mixture_reverse(mr, p, y, T, x, cb, 0.0, Hab, 0.0, false, mb, Tb, ybar, lane, mem);
     wave_sync();
 }
@@ -289,7 +341,7 @@ __device__ __forceinline__ void t16_vjp_f(const DeviceMech& mr, T16AdjLDS<N>& L,
 template <int N>
 __device__ __forceinline__ void t16_jact(const DeviceMech& mr, T16AdjLDS<N>& L, double p,
                                          double y, double F, double Fb, double& ybar,
-                                         const adj::ThView& th, int lane, int mem) {
+                                         const T16Th& th, int lane, int mem) {
     constexpr int P = T16AdjLDS<N>::P;
     const int nS = mr.nS, iT = nS;
     const double T = team_bcast(y, lane, nS);
@@ -390,6 +442,7 @@ __device__ __forceinline__ void t16_jact(const DeviceMech& mr, T16AdjLDS<N>& L, 
     }
     L.cb[mem] = sp ? cbT : 0.0;
     wave_sync();
+    t16_sets_begin<N>(mr, L, mem);
     // reactions (gpuReaction.H:547-702): adjoints of omega, dOmega/dT,
     // kf dCf/dc_j, kr dCr/dc_j and D from the raw-entry adjoints
     double Tb = 0.0;
@@ -404,14 +457,26 @@ __device__ __forceinline__ void t16_jact(const DeviceMech& mr, T16AdjLDS<N>& L, 
         in.fast = true;
         auto u = [&](int col) {
             double v = 0.0;
-            for (int q = 0; q < kTeamWidth && q < nl; ++q) v -= mr.ls[o + q] * L.Jb()[mr.li[o + q] * P + col];
-            for (int q = 0; q < kTeamWidth && q < nr; ++q) v += mr.rs[o + q] * L.Jb()[mr.ri[o + q] * P + col];
+#pragma unroll
+            for (int q = 0; q < kTeamWidth; ++q)
+                if (q < nl) v -= mr.ls[o + q] * L.Jb()[mr.li[o + q] * P + col];
+#pragma unroll
+            for (int q = 0; q < kTeamWidth; ++q)
+                if (q < nr) v += mr.rs[o + q] * L.Jb()[mr.ri[o + q] * P + col];
             return v;
         };
-        for (int q = 0; q < kTeamWidth && q < nl; ++q) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
-        for (int q = 0; q < kTeamWidth && q < nr; ++q) in.w += mr.rs[o + q] * L.dNb[mr.ri[o + q]];
-        for (int j = 0; j < kTeamWidth && j < nl; ++j) in.dCf[j] = u(mr.li[o + j]);
-        for (int j = 0; j < kTeamWidth && j < nr; ++j) in.dCr[j] = -u(mr.ri[o + j]);
+#pragma unroll
+        for (int q = 0; q < kTeamWidth; ++q)
+            if (q < nl) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
+#pragma unroll
+        for (int q = 0; q < kTeamWidth; ++q)
+            if (q < nr) in.w += mr.rs[o + q] * L.dNb[mr.ri[o + q]];
+#pragma unroll
+        for (int j = 0; j < kTeamWidth; ++j)
+            if (j < nl) in.dCf[j] = u(mr.li[o + j]);
+#pragma unroll
+        for (int j = 0; j < kTeamWidth; ++j)
+            if (j < nr) in.dCr[j] = -u(mr.ri[o + j]);
         in.dwdT = u(iT);
         if (mr.type[r] == GFC_RATE_THIRD_BODY) {
             const double* eff = mr.eff + (long)r * nS;
@@ -421,13 +486,12 @@ __device__ __forceinline__ void t16_jact(const DeviceMech& mr, T16AdjLDS<N>& L, 
             in.D = D;
         }
         double t6[adj::kParams] = {0, 0, 0, 0, 0, 0};
-        adj::reaction_reverse_w<kTeamWidth>(mr, r, T, false, lane::SV{L.f.c, 1}, in,
-                                            adj::LdsAcc{L.cb}, Tb, t6);
-        th.add(mr.rxnRef[r], t6);
+        t16_reaction_reverse<N>(mr, L, r, T, false, in, Tb, t6);
+        th.add(pass, mem, t6);
     }
     Tb = team_sum(Tb, lane);
     wave_sync();
-    const double cb = sp ? L.cb[mem] : 0.0;
+    const double cb = sp ? L.cb[mem] + t16_sets_scatter<N>(mr, L, mem) : 0.0;
     t16_mixture_reverse(mr, p, y, T, x, cb, Cpb, Hab, vb, true, mb, Tb, ybar, lane, mem);
     wave_sync();
 }
@@ -438,7 +502,7 @@ __device__ __forceinline__ void t16_jact(const DeviceMech& mr, T16AdjLDS<N>& L, 
 template <int N>
 __device__ __forceinline__ double t16_step_reverse(const SolveArgs& A, const DeviceMech& mr,
                                                    T16AdjLDS<N>& L, double p, double y0,
-                                                   double h, double yb, const adj::ThView& th,
+                                                   double h, double yb, const T16Th& th,
                                                    int lane, int mem) {
     const DeviceMech& m = A.m;
     const int nS = m.nS, n = nS + 1;
@@ -506,12 +570,14 @@ __device__ __forceinline__ double t16_step_reverse(const SolveArgs& A, const Dev
 // Reverse sweep, one team per cell (adjoint.hip adjoint_reverse_kernel).
 // A.m: the team solve's tables.  seg: the K-state segment buffers (segPer
 // doubles per cell, wave-blocked).  Ybar/Y0bar: [n][ldb]; thBar:
-// [6 nR][ldb], reactions in reference order.
+// [6 nR][ldb], reactions in reference order.  pacc: the parameter
+// accumulators, cell c at pacc + (c/64)*paccSlot + (c%64)*paccPer, each
+// kParams*16*nPass doubles long.
 template <int N>
 __global__ void __launch_bounds__(64, 2)
 team16_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer,
                               const double* Ybar, long ldb, double* Y0bar, double* thBar,
-                              int* queue) {
+                              double* pacc, long paccSlot, long paccPer, int* queue) {
     __shared__ T16AdjLDS<N> Ls[kLanes / kTW];
     const int lane = threadIdx.x;
     const int team = lane >> 4, mem = lane & 15;
@@ -527,9 +593,11 @@ team16_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer
         const double p = y0c[(long)n * kRecStride];
         const double T0 = y0c[(long)nS * kRecStride];
         double yb = (mem < n) ? Ybar[mem * ldb + cell] : 0.0;
-        const adj::ThView th{thBar ? thBar + cell : nullptr, ldb};
-        if (th.p)
-            for (long q = mem; q < (long)adj::kParams * mr.nR; q += kTW) th.p[q * ldb] = 0.0;
+        const int nPass = (mr.nR + kTW - 1) / kTW;
+        const T16Th th{thBar ? pacc + (cell / kLanes) * paccSlot + (cell % kLanes) * paccPer
+                             : nullptr};
+        if (th.acc)
+            for (int q = mem; q < adj::kParams * kTW * nPass; q += kTW) th.acc[q] = 0.0;
         if (T0 > A.Treact && A.deltaT > kSmall) {
             const double* yE = rec_slot(R.yEnd, cell, n);
             if (mem < nS && yE[mem * kRecStride] < 0.0) yb = 0.0;
@@ -555,15 +623,25 @@ team16_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer
             }
         }
         if (mem < n) Y0bar[mem * ldb + cell] = yb;
+        if (th.acc)
+            for (int pass = 0; pass < nPass; ++pass) {
+                const int r = pass * kTW + mem;
+                if (r >= mr.nR) break;
+                const long ref = mr.rxnRef[r];
+#pragma unroll
+                for (int q = 0; q < adj::kParams; ++q)
+                    thBar[(adj::kParams * ref + q) * ldb + cell] =
+                        th.acc[(pass * adj::kParams + q) * kTW + mem];
+            }
         wave_sync();
     }
 }
 
 template __global__ void team16_adjoint_reverse_kernel<11>(SolveArgs, T16Record, double*, long,
                                                            const double*, long, double*,
-                                                           double*, int*);
+                                                           double*, double*, long, long, int*);
 template __global__ void team16_adjoint_reverse_kernel<16>(SolveArgs, T16Record, double*, long,
                                                            const double*, long, double*,
-                                                           double*, int*);
+                                                           double*, double*, long, long, int*);
 
 } // namespace gfc

@@ -64,14 +64,16 @@ struct T16AdjLDS {
 // contiguous per-cell block in the order the sweeps visit the reactions,
 // acc[(pass*kParams + q)*16 + mem] for reaction pass*16 + mem, so a team's
 // additions are coalesced; written to thBar in reference order when the
-// cell ends.
+// cell ends.  Every access is an atomic at device scope, so the additions
+// need not wait for the value they update (no-return adds at the L2) and
+// the final read sees them.
 struct T16Th {
     double* acc; // nullptr: no parameter adjoints
     __device__ __forceinline__ void add(int pass, int mem, const double* v) const {
         if (!acc) return;
 #pragma unroll
         for (int q = 0; q < adj::kParams; ++q)
-            if (v[q] != 0.0) acc[(pass * adj::kParams + q) * kTW + mem] += v[q];
+            if (v[q] != 0.0) atomicAdd(&acc[(pass * adj::kParams + q) * kTW + mem], v[q]);
     }
 };
 
@@ -597,7 +599,8 @@ team16_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer
         const T16Th th{thBar ? pacc + (cell / kLanes) * paccSlot + (cell % kLanes) * paccPer
                              : nullptr};
         if (th.acc)
-            for (int q = mem; q < adj::kParams * kTW * nPass; q += kTW) th.acc[q] = 0.0;
+            for (int q = mem; q < adj::kParams * kTW * nPass; q += kTW)
+                __hip_atomic_store(&th.acc[q], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (T0 > A.Treact && A.deltaT > kSmall) {
             const double* yE = rec_slot(R.yEnd, cell, n);
             if (mem < nS && yE[mem * kRecStride] < 0.0) yb = 0.0;
@@ -631,7 +634,8 @@ team16_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer
 #pragma unroll
                 for (int q = 0; q < adj::kParams; ++q)
                     thBar[(adj::kParams * ref + q) * ldb + cell] =
-                        th.acc[(pass * adj::kParams + q) * kTW + mem];
+                        __hip_atomic_load(&th.acc[(pass * adj::kParams + q) * kTW + mem],
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         wave_sync();
     }

@@ -54,14 +54,16 @@ namespace {
 // contiguous per-cell block in the order the sweeps visit the reactions,
 // acc[(pass*kParams + q)*64 + lane] for reaction pass*64 + lane, so each
 // sweep's additions are coalesced; written to thBar in reference order
-// when the cell ends.
+// when the cell ends.  Every access is an atomic at device scope, so the
+// additions need not wait for the value they update (no-return adds at
+// the L2) and the final read sees them.
 struct WTh {
     double* acc; // nullptr: no parameter adjoints
     __device__ __forceinline__ void add(int pass, int lane, const double* v) const {
         if (!acc) return;
 #pragma unroll
         for (int q = 0; q < adj::kParams; ++q)
-            if (v[q] != 0.0) acc[(pass * adj::kParams + q) * kLanes + lane] += v[q];
+            if (v[q] != 0.0) atomicAdd(&acc[(pass * adj::kParams + q) * kLanes + lane], v[q]);
     }
 };
 
@@ -731,7 +733,8 @@ wave_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer,
         const WTh th{thBar ? pacc + (cell / kLanes) * paccSlot + (cell % kLanes) * paccPer
                            : nullptr};
         if (th.acc)
-            for (int q = lane; q < nAccP; q += kLanes) th.acc[q] = 0.0;
+            for (int q = lane; q < nAccP; q += kLanes)
+                __hip_atomic_store(&th.acc[q], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (T0 > A.Treact && A.deltaT > kSmall) {
             const double* yE = rec_slot(R.yEnd, cell, n);
             if (lane < nS && yE[lane * kRecStride] < 0.0) yb = 0.0;
@@ -765,7 +768,8 @@ wave_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer,
 #pragma unroll
                 for (int q = 0; q < adj::kParams; ++q)
                     thBar[(adj::kParams * ref + q) * ldb + cell] =
-                        th.acc[(pass * adj::kParams + q) * kLanes + lane];
+                        __hip_atomic_load(&th.acc[(pass * adj::kParams + q) * kLanes + lane],
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         wave_sync();
     }

@@ -101,6 +101,7 @@ struct WStages {
     double F, dydx0, k1, k2, k3, k4, f2, f3;
     int ps;       // pivot position of this lane's row
     double col[N]; // column `lane` of the LU factors, rows in pivot order
+    double rdg;    // 1 / U[lane][lane] (1 on padding lanes)
 };
 
 // The accepted attempt of a step of size dx from y, as the solve computes it
@@ -133,6 +134,7 @@ __device__ __forceinline__ double w_step(const SolveArgs& A, WaveAdjLDS<N>& L, d
     double rd;
     team_lu<N>(row, n, lane, ps, rd, plv);
     if constexpr (KEEP) {
+        S.rdg = 1.0;
         // BR pivot positions at a time: their rows into the weight region,
         // then each lane takes its column (a NaN Jacobian can leave a row at
         // a padding position >= n)
@@ -149,10 +151,14 @@ __device__ __forceinline__ double w_step(const SolveArgs& A, WaveAdjLDS<N>& L, d
 #pragma unroll
             for (int t = 0; t < BR; ++t) {
                 const int j = BR * b + t;
-                if (j < N) S.col[j] = (j < n && lane < n) ? X[t * P + min(lane, N - 1)] : 0.0;
+                if (j < N) {
+                    S.col[j] = (j < n && lane < n) ? X[t * P + min(lane, N - 1)] : 0.0;
+                    if (j == lane) S.rdg = S.col[j];
+                }
             }
             wave_sync();
         }
+        S.rdg = (lane < n) ? 1.0 / S.rdg : 1.0;
     }
     double k1 = 0, k2 = 0, k3 = 0, k4 = 0, dydx = 0, f2 = 0, f3 = 0;
 #pragma unroll 1
@@ -191,16 +197,16 @@ __device__ __forceinline__ double w_step(const SolveArgs& A, WaveAdjLDS<N>& L, d
 
 // Solves A^T x = r for the step's A = P^T L U (adjoint_math.hpp lu_solve_T):
 // lane i holds r_i and receives x_i; lane k holds column k of the factors
-// (col[j] = LU[j][k], rows in pivot order).
+// (col[j] = LU[j][k], rows in pivot order) and rdg = 1/U[k][k].
 template <int N>
-__device__ __forceinline__ double w_solveT(const double (&col)[N], int n, int ps, double r,
-                                           int lane) {
+__device__ __forceinline__ double w_solveT(const double (&col)[N], double rdg, int n, int ps,
+                                           double r, int lane) {
     // U^T w = r
     double w = 0.0;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         if (j < n) {
-            const double wj = readlane(r, j) / readlane(col[j], j);
+            const double wj = readlane(r, j) * readlane(rdg, j);
             if (lane == j) w = wj;
             if (lane > j && lane < n) r -= col[j] * wj;
         }
@@ -456,8 +462,8 @@ __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, d
         if (sp) {
             const double* gT = jb_b(L, rank);
             double h = 0.0;
-            for (int i = 0; i < nS; ++i) h += -gT[i] * L.f.J[lane * P + i] / CpM;
-            Hab += h;
+            for (int i = 0; i < nS; ++i) h += gT[i] * L.f.J[lane * P + i];
+            Hab += -h / CpM;
         }
         ++rank;
         double fbT = 0.0, cpm = 0.0;
@@ -518,9 +524,10 @@ __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, d
             avPart = g * (F - WiR * sumc);
             rhoPart += -g * mr.W[lane] / (rho * rho) * X;
             Fb += g * av;
+            const double rW = rho / mr.W[lane];
             for (int q = 0; q < rank; ++q) {
                 jb_a(L, q)[lane] *= WiR;
-                jb_b(L, q)[lane] *= rho / mr.W[lane];
+                jb_b(L, q)[lane] *= rW;
             }
             jb_a(L, rank)[lane] = -g * WiR * av;
             jb_b(L, rank)[lane] = L.f.c[lane];
@@ -654,7 +661,7 @@ __device__ __forceinline__ double w_step_reverse(const SolveArgs& A, const Devic
     double Fb = 0.0, fb2 = 0.0, fb3 = 0.0;
     int rank = 0; // factor pairs of Jb
     if (t.stages == 4) {
-        const double rb = w_solveT<N>(S.col, n, S.ps, kb4, lane);
+        const double rb = w_solveT<N>(S.col, S.rdg, n, S.ps, kb4, lane);
         w_add_outer<N>(L, n, rb, S.k4, lane, rank);
         fb3 += rb;
         Fb += h * t.d4 * rb;
@@ -663,7 +670,7 @@ __device__ __forceinline__ double w_step_reverse(const SolveArgs& A, const Devic
         kb3 += t.c43 / h * rb;
     }
     if (t.stages >= 3) {
-        const double rb = w_solveT<N>(S.col, n, S.ps, kb3, lane);
+        const double rb = w_solveT<N>(S.col, S.rdg, n, S.ps, kb3, lane);
         w_add_outer<N>(L, n, rb, S.k3, lane, rank);
         if (t.stages == 4) fb3 += rb;
         else fb2 += rb;
@@ -680,7 +687,7 @@ __device__ __forceinline__ double w_step_reverse(const SolveArgs& A, const Devic
         kb2 += t.a32 * ysb;
     }
     {
-        const double rb = w_solveT<N>(S.col, n, S.ps, kb2, lane);
+        const double rb = w_solveT<N>(S.col, S.rdg, n, S.ps, kb2, lane);
         w_add_outer<N>(L, n, rb, S.k2, lane, rank);
         fb2 += rb;
         Fb += h * t.d2 * rb;
@@ -693,7 +700,7 @@ __device__ __forceinline__ double w_step_reverse(const SolveArgs& A, const Devic
         yb += ysb;
         kb1 += t.a21 * ysb;
     }
-    const double rb = w_solveT<N>(S.col, n, S.ps, kb1, lane);
+    const double rb = w_solveT<N>(S.col, S.rdg, n, S.ps, kb1, lane);
     w_add_outer<N>(L, n, rb, S.k1, lane, rank);
     Fb += h * t.d1 * rb;
     // within the reaction temperature limits f(y0) is the Jacobian call's F

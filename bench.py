@@ -22,7 +22,8 @@ Printed (one JSON line): the headline metric and, beside it,
     over a bounded sample of the same cells, and the error of the GPU
     result against that CPU result on the sample;
   * the adjoint/forward cost on the differentiation workload (10-species
-    H2, 65536 cells, deltaT 1e-4).
+    H2, 65536 cells, deltaT 1e-4) and on 16384 cells of the 53-species
+    workload.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--config gri_production]
 """
@@ -217,6 +218,8 @@ def main():
                     help="cells of the differentiation workload for the adjoint cost (0: skip)")
     ap.add_argument("--adjoint-config", default="h2_adjoint",
                     help="workload of the adjoint cost (default: the differentiation workload)")
+    ap.add_argument("--adjoint-gri-cells", type=int, default=16384,
+                    help="cells of the 53-species workload for a second adjoint cost (0: skip)")
     ap.add_argument("--adjoint-dt", type=float, default=0.0,
                     help="CFD step of the adjoint cost in place of the workload's (0: keep)")
     ap.add_argument("--autotune", type=int, default=0, metavar="CELLS",
@@ -349,6 +352,10 @@ def main():
     if args.adjoint_cells:
         progress(f"adjoint cost on {args.adjoint_cells} cells of {args.adjoint_config}")
         adjoint = adjoint_cost(args.adjoint_cells, args.adjoint_config, args.adjoint_dt)
+    adjoint_gri = None
+    if args.adjoint_gri_cells:
+        progress(f"adjoint cost on {args.adjoint_gri_cells} cells of gri_production")
+        adjoint_gri = adjoint_cost(args.adjoint_gri_cells, "gri_production")
 
     batching = None
     if args.batching_cells:
@@ -404,6 +411,7 @@ def main():
         "batched_over_lane": value / lane_value if lane_value else None,
         "batching": batching,
         "adjoint": adjoint,
+        "adjoint_gri": adjoint_gri,
         "autotune": tuned,
         "attempts_per_cell": float(st[1].mean()),
         "accepted_per_cell": float(st[2].mean()),

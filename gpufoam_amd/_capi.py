@@ -10,7 +10,11 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgpufoam_amd.so")
+# The debug bounds-check build of the same sources (include/gpufoam_amd.h,
+# gfc_bounds_checked); the environment variable GFC_LIBRARY selects a
+# library file in place of the default one.
+BOUNDS_LIB_PATH = os.path.join(_HERE, "libgpufoam_amd_bounds.so")
+LIB_PATH = os.environ.get("GFC_LIBRARY") or os.path.join(_HERE, "libgpufoam_amd.so")
 
 MAX_REACTION_WIDTH = 10
 NO_INT_EXPONENT = 214748364
@@ -19,7 +23,7 @@ GFC_OK = 0
 STATUS_NAMES = {
     1: "INVALID_ARGUMENT", 2: "PARSE", 3: "UNSUPPORTED", 4: "MASS_IMBALANCE",
     5: "TOO_LONG", 6: "TOO_MANY_SPECIES", 7: "EFFICIENCIES", 8: "SOLVER_NAME",
-    9: "CAPACITY", 10: "HIP", 11: "NOT_BUILT", 12: "LTS",
+    9: "CAPACITY", 10: "HIP", 11: "NOT_BUILT", 12: "LTS", 13: "BOUNDS",
 }
 
 RATE_ARRHENIUS, RATE_THIRD_BODY, RATE_LINDEMANN, RATE_TROE = 0, 1, 2, 3
@@ -93,6 +97,7 @@ def lib() -> C.CDLL:
         vp = C.c_void_p
         sigs = {
             "gfc_last_error": (C.c_char_p, []),
+            "gfc_bounds_checked": (C.c_int, []),
             "gfc_mechanism_from_foam": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(vp)]),
             "gfc_mechanism_destroy": (None, [vp]),
             "gfc_chemkin_to_foam": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t,
@@ -154,7 +159,8 @@ def lib() -> C.CDLL:
 def exported_symbols() -> list[str]:
     """Names declared in include/gpufoam_amd.h (checked by the CPU tests)."""
     return [
-        "gfc_last_error", "gfc_mechanism_from_foam", "gfc_mechanism_destroy",
+        "gfc_last_error", "gfc_bounds_checked", "gfc_mechanism_from_foam",
+        "gfc_mechanism_destroy",
         "gfc_mechanism_view", "gfc_mechanism_species_name",
         "gfc_mechanism_reaction_equation", "gfc_mechanism_raw_thermo",
         "gfc_mechanism_species_index", "gfc_ode_inputs_from_foam",

@@ -493,7 +493,7 @@ int gfc_adjoint_forward(gfc_evaluator* ev, double deltaT, int nCells, int ldY,
     st->nCells = nCells;
     st->deltaT = deltaT;
     st->valid = true;
-    return GFC_OK;
+    return bounds_report(ev, s);
 }
 
 int gfc_adjoint_backward(gfc_evaluator* ev, const double* Ybar, int ldb, double* Y0bar,
@@ -542,7 +542,7 @@ int gfc_adjoint_backward(gfc_evaluator* ev, const double* Ybar, int ldb, double*
                            (long)ldb, Y0bar, paramBar);
     }
     if (hipFail(hipGetLastError(), "adjoint reverse launch")) return GFC_ERR_HIP;
-    return GFC_OK;
+    return bounds_report(ev, s);
 }
 
 int gfc_adjoint_replay(gfc_evaluator* ev, const double* Y0, int ldY, double* Ynew,
@@ -566,7 +566,7 @@ int gfc_adjoint_replay(gfc_evaluator* ev, const double* Y0, int ldY, double* Yne
     hipLaunchKernelGGL(adjoint_replay_kernel, dim3(blocks), dim3(64), 0, s, A, layout(ev),
                        st->scratch, st->hs, st->nAcc, st->y0);
     if (hipFail(hipGetLastError(), "adjoint replay launch")) return GFC_ERR_HIP;
-    return GFC_OK;
+    return bounds_report(ev, s);
 }
 
 } // extern "C"

@@ -1165,6 +1165,7 @@ block_solve_kernel(SolveArgs A, int* queue, double* scratch, long perWG) {
             next += gridDim.x;
         }
         if (cell >= A.nCells) break;
+        cell = bound(cell, A.nCells, m.bounds, kBoundCell);
         if (t < n) L.y[t] = A.Y0[t * ld + cell];
         const double p = A.Y0[(long)(nS + 1) * ld + cell];
         double dtChem = A.dtChem[cell];

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/gpufoam_amd.h"
+#include "bounds.hpp"
 #include "constants.hpp"
 
 namespace gfc {
@@ -104,6 +105,9 @@ struct DeviceMech {
     // kernels evaluate them)
     const double* extra;
     int extended;
+    // the bounds-check word of a library built with GFC_BOUNDS_CHECK (null
+    // otherwise)
+    unsigned* bounds;
 };
 
 // Rates that depend on a third-body concentration M.

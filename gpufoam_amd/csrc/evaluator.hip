@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <type_traits>
 #include <cstring>
 #include <string>
@@ -380,7 +381,26 @@ int evaluator_upload(gfc_evaluator* ev) {
                      [&](int a, int b) { return rank(a) < rank(b); });
     st = upload_tables(v, perm, ev->dmTeam, ev->mechBlockTeam, block_eligible(v));
     if (st != GFC_OK) return st;
+    ev->dm.bounds = ev->dmTeam.bounds = ev->boundsWord;
     return GFC_OK;
+}
+
+int bounds_report(gfc_evaluator* ev, hipStream_t s) {
+    if (!kBoundsChecked || !ev->boundsWord) return GFC_OK;
+    unsigned w = 0;
+    if (hipFail(hipMemcpyAsync(&w, ev->boundsWord, sizeof w, hipMemcpyDeviceToHost, s),
+                "hipMemcpyAsync(bounds)") ||
+        hipFail(hipStreamSynchronize(s), "hipStreamSynchronize"))
+        return GFC_ERR_HIP;
+    if (w == 0) return GFC_OK;
+    const unsigned z = 0;
+    (void)hipMemcpy(ev->boundsWord, &z, sizeof z, hipMemcpyHostToDevice);
+    char msg[160];
+    snprintf(msg, sizeof msg,
+             "bounds check failed (mask 0x%x: 1 specie, 2 efficiency set, 4 cell, "
+             "8 recorded step, 16 checkpoint slot)", w);
+    setLastError(msg);
+    return GFC_ERR_BOUNDS;
 }
 
 } // namespace gfc
@@ -414,6 +434,13 @@ int gfc_evaluator_create(const gfc_mechanism* mech, const gfc_ode_inputs* ode,
         return sst;
     }
     ev->mech = *mech;
+    if (kBoundsChecked) {
+        if (hipFail(hipMalloc(&ev->boundsWord, sizeof(unsigned)), "hipMalloc(bounds)") ||
+            hipFail(hipMemset(ev->boundsWord, 0, sizeof(unsigned)), "hipMemset(bounds)")) {
+            gfc_evaluator_destroy(ev);
+            return GFC_ERR_HIP;
+        }
+    }
     int st = evaluator_upload(ev);
     if (st != GFC_OK) {
         gfc_evaluator_destroy(ev);
@@ -452,7 +479,7 @@ void gfc_evaluator_destroy(gfc_evaluator* ev) {
     adjoint_free(ev);
     void* ptrs[] = {ev->mechBlock, ev->mechBlockTeam, ev->laneScratch, ev->minBits, ev->sY0,
                     ev->sYnew, ev->sRR, ev->sRho, ev->sDt, ev->sDtMin,
-                    ev->sStats};
+                    ev->sStats, ev->boundsWord};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete ev;
@@ -557,7 +584,10 @@ int gfc_solve(gfc_evaluator* ev, double deltaT, double deltaTChemMax,
                                sizeof(int) * nCells, hipMemcpyDeviceToHost, s);
         if (hipFail(hipStreamSynchronize(s), "hipStreamSynchronize")) return GFC_ERR_HIP;
     }
+    if (st == GFC_OK) st = bounds_report(ev, s);
     return st;
 }
+
+int gfc_bounds_checked(void) { return kBoundsChecked ? 1 : 0; }
 
 } // extern "C"

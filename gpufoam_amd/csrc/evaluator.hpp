@@ -24,6 +24,7 @@ struct gfc_evaluator {
     long lanePer = 0;     // scratch doubles per cell (wave-blocked layout)
     size_t laneScratchBytes = 0;
     unsigned long long* minBits = nullptr;
+    unsigned* boundsWord = nullptr; // bounds-check build: failed-check bits
     // staging for GFC_HOST_POINTERS
     double* sY0 = nullptr;
     double* sYnew = nullptr;
@@ -46,6 +47,10 @@ namespace gfc {
 SolveArgs solve_args(const gfc_evaluator* ev, double deltaT, double deltaTChemMax,
                      int nCells);
 int evaluator_upload(gfc_evaluator* ev);
+// Bounds-check build: waits for the stream and returns GFC_ERR_BOUNDS (and
+// clears the word) when a kernel's check failed; GFC_OK otherwise and in the
+// default build, without waiting.
+int bounds_report(gfc_evaluator* ev, hipStream_t s);
 bool block_eligible(const gfc_mech_view& v);
 void adjoint_free(gfc_evaluator* ev);
 int batched_init(gfc_evaluator* ev);

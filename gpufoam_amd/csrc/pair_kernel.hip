@@ -624,6 +624,7 @@ pair_solve_kernel(SolveArgs A, int* queue) {
             cBase = cell;
         }
         if (cell >= A.nCells) break;
+        cell = bound(cell, A.nCells, A.m.bounds, kBoundCell);
 
         double y = (lane < n) ? A.Y0[lane * ld + cell] : 0.0;
         const double y0cell = y;

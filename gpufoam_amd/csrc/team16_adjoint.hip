@@ -38,6 +38,11 @@ __device__ __forceinline__ void acc_add(LdsAcc v, int i, double x) { atomicAdd(&
 
 namespace {
 
+// A specie index read from the tables (checked in a bounds-check build).
+__device__ __forceinline__ int spx(const DeviceMech& m, int i) {
+    return bound(i, m.nS, m.bounds, kBoundSpecies);
+}
+
 using namespace t16;
 
 template <int N>
@@ -321,10 +326,10 @@ __device__ __forceinline__ void t16_vjp_f(const DeviceMech& mr, T16AdjLDS<N>& L,
         const int nl = mr.nl[r], nr = mr.nr[r];
 #pragma unroll
         for (int q = 0; q < kTeamWidth; ++q)
-            if (q < nl) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
+            if (q < nl) in.w -= mr.ls[o + q] * L.dNb[spx(mr, mr.li[o + q])];
 #pragma unroll
         for (int q = 0; q < kTeamWidth; ++q)
-            if (q < nr) in.w += mr.rs[o + q] * L.dNb[mr.ri[o + q]];
+            if (q < nr) in.w += mr.rs[o + q] * L.dNb[spx(mr, mr.ri[o + q])];
         if (in.w == 0.0) continue;
         double t6[adj::kParams] = {0, 0, 0, 0, 0, 0};
         t16_reaction_reverse<N>(mr, L, r, T, true, in, Tb, t6);
@@ -461,24 +466,24 @@ __device__ __forceinline__ void t16_jact(const DeviceMech& mr, T16AdjLDS<N>& L, 
             double v = 0.0;
 #pragma unroll
             for (int q = 0; q < kTeamWidth; ++q)
-                if (q < nl) v -= mr.ls[o + q] * L.Jb()[mr.li[o + q] * P + col];
+                if (q < nl) v -= mr.ls[o + q] * L.Jb()[spx(mr, mr.li[o + q]) * P + col];
 #pragma unroll
             for (int q = 0; q < kTeamWidth; ++q)
-                if (q < nr) v += mr.rs[o + q] * L.Jb()[mr.ri[o + q] * P + col];
+                if (q < nr) v += mr.rs[o + q] * L.Jb()[spx(mr, mr.ri[o + q]) * P + col];
             return v;
         };
 #pragma unroll
         for (int q = 0; q < kTeamWidth; ++q)
-            if (q < nl) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
+            if (q < nl) in.w -= mr.ls[o + q] * L.dNb[spx(mr, mr.li[o + q])];
 #pragma unroll
         for (int q = 0; q < kTeamWidth; ++q)
-            if (q < nr) in.w += mr.rs[o + q] * L.dNb[mr.ri[o + q]];
+            if (q < nr) in.w += mr.rs[o + q] * L.dNb[spx(mr, mr.ri[o + q])];
 #pragma unroll
         for (int j = 0; j < kTeamWidth; ++j)
-            if (j < nl) in.dCf[j] = u(mr.li[o + j]);
+            if (j < nl) in.dCf[j] = u(spx(mr, mr.li[o + j]));
 #pragma unroll
         for (int j = 0; j < kTeamWidth; ++j)
-            if (j < nr) in.dCr[j] = -u(mr.ri[o + j]);
+            if (j < nr) in.dCr[j] = -u(spx(mr, mr.ri[o + j]));
         in.dwdT = u(iT);
         if (mr.type[r] == GFC_RATE_THIRD_BODY) {
             const double* eff = mr.eff + (long)r * nS;
@@ -589,8 +594,9 @@ team16_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer
     for (;;) {
         int c = 0;
         if (mem == 0) c = atomicAdd(queue, 1);
-        const int cell = team_bcast_i(c, lane, 0);
+        int cell = team_bcast_i(c, lane, 0);
         if (cell >= A.nCells) break;
+        cell = bound(cell, A.nCells, mr.bounds, kBoundCell);
         const double* y0c = rec_slot(R.y0c, cell, n + 1);
         const double p = y0c[(long)n * kRecStride];
         const double T0 = y0c[(long)nS * kRecStride];
@@ -604,14 +610,16 @@ team16_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer
         if (T0 > A.Treact && A.deltaT > kSmall) {
             const double* yE = rec_slot(R.yEnd, cell, n);
             if (mem < nS && yE[mem * kRecStride] < 0.0) yb = 0.0;
-            const int nAcc = R.nAcc[cell];
+            const int nAcc = bound(R.nAcc[cell], R.maxSteps + 1, mr.bounds, kBoundStep);
             const double* ck = rec_slot(R.ck, cell, R.nck * n);
             const double* hs = rec_slot(R.hs, cell, R.maxSteps);
             double* sg = rec_slot(seg, cell, segPer);
             const int nseg = (nAcc + R.K - 1) / R.K;
             for (int g = nseg - 1; g >= 0; --g) {
                 const int s0 = g * R.K, s1 = min(s0 + R.K, nAcc);
-                double y = (mem < n) ? ck[((long)g * n + mem) * kRecStride] : 0.0;
+                double y = (mem < n) ? ck[((long)bound(g, R.nck, mr.bounds, kBoundSlot) * n + mem) *
+                                             kRecStride]
+                                        : 0.0;
                 if (mem < n) sg[mem * kRecStride] = y;
                 T16Stages S;
                 for (int t = s0; t < s1 - 1; ++t) {

@@ -60,6 +60,7 @@ __device__ __forceinline__ void t16_run(const SolveArgs& A, int* queue, const T1
             cell = (int)blockIdx.x * (kLanes / kTW) + team;
         }
         if (cell >= A.nCells) break;
+        cell = bound(cell, A.nCells, A.m.bounds, kBoundCell);
 
         double y = (mem < n) ? A.Y0[mem * ld + cell] : 0.0;
         const double y0cell = y;

@@ -126,6 +126,14 @@ __device__ __forceinline__ Rxn load_rxn(const DeviceMech& m, int pass, int lane)
     for (int q = 0; q < 4; ++q) x.troe[q] = D[(rdAlpha + q) * kLanes];
     x.dn = D[rdDn * kLanes];
     x.Tcommon = D[rdTcommon * kLanes];
+    if (kBoundsChecked) {
+#pragma unroll
+        for (int q = 0; q < kTeamWidth; ++q) {
+            if (q < x.nl) x.li[q] = bound(x.li[q], m.nS, m.bounds, kBoundSpecies);
+            if (q < x.nr) x.ri[q] = bound(x.ri[q], m.nS, m.bounds, kBoundSpecies);
+        }
+        if (x.set >= 0) x.set = bound(x.set, m.nSets, m.bounds, kBoundSet);
+    }
     return x;
 }
 

@@ -80,6 +80,7 @@ __device__ __forceinline__ void team_run(const SolveArgs& A, int* queue, const t
             cBase = cell;
         }
         if (cell >= A.nCells) break;
+        cell = bound(cell, A.nCells, m.bounds, kBoundCell);
 
         double y = (lane < n) ? A.Y0[lane * ld + cell] : 0.0;
         const double y0cell = y;
@@ -185,11 +186,13 @@ __device__ __forceinline__ void team_run(const SolveArgs& A, int* queue, const t
                     ++accepted;
                     if constexpr (REC) {
                         // the state at the start of the step and its size
+                        const int ct = bound(count, R.maxSteps, m.bounds, kBoundStep);
                         if (lane == 0)
-                            t16::rec_slot(R.hs, cell, R.maxSteps)[(long)count * t16::kRecStride] = dx;
+                            t16::rec_slot(R.hs, cell, R.maxSteps)[(long)ct * t16::kRecStride] = dx;
                         if (count % R.K == 0 && lane < n)
-                            t16::rec_slot(R.ck, cell, R.nck * n)[((long)(count / R.K) * n + lane) *
-                                                                 t16::kRecStride] = y;
+                            t16::rec_slot(R.ck, cell, R.nck * n)
+                                [((long)bound(count / R.K, R.nck, m.bounds, kBoundSlot) * n + lane) *
+                                 t16::kRecStride] = y;
                         ++count;
                     }
                     x += dx;

@@ -50,6 +50,11 @@ __device__ __forceinline__ void acc_add(WaveAcc v, int i, double x) { atomicAdd(
 
 namespace {
 
+// A specie index read from the tables (checked in a bounds-check build).
+__device__ __forceinline__ int spx(const DeviceMech& m, int i) {
+    return bound(i, m.nS, m.bounds, kBoundSpecies);
+}
+
 // Rate-parameter adjoints of one cell while its steps are reversed: a
 // contiguous per-cell block in the order the sweeps visit the reactions,
 // acc[(pass*kParams + q)*64 + lane] for reaction pass*64 + lane, so each
@@ -398,10 +403,10 @@ __device__ __forceinline__ void w_vjp_f(const DeviceMech& mr, WaveAdjLDS<N>& L, 
         const int nl = mr.nl[r], nr = mr.nr[r];
 #pragma unroll
         for (int q = 0; q < kTeamWidth; ++q)
-            if (q < nl) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
+            if (q < nl) in.w -= mr.ls[o + q] * L.dNb[spx(mr, mr.li[o + q])];
 #pragma unroll
         for (int q = 0; q < kTeamWidth; ++q)
-            if (q < nr) in.w += mr.rs[o + q] * L.dNb[mr.ri[o + q]];
+            if (q < nr) in.w += mr.rs[o + q] * L.dNb[spx(mr, mr.ri[o + q])];
         if (in.w == 0.0) continue;
         double t6[adj::kParams] = {0, 0, 0, 0, 0, 0};
         w_reaction_reverse<N>(mr, L, r, T, true, in, Tb, t6);
@@ -592,8 +597,8 @@ __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, d
             double v = 0.0;
             if (q < rank) {
                 const double* aq = jb_a(L, q);
-                for (int t = 0; t < kTeamWidth && t < nl; ++t) v -= mr.ls[o + t] * aq[mr.li[o + t]];
-                for (int t = 0; t < kTeamWidth && t < nr; ++t) v += mr.rs[o + t] * aq[mr.ri[o + t]];
+                for (int t = 0; t < kTeamWidth && t < nl; ++t) v -= mr.ls[o + t] * aq[spx(mr, mr.li[o + t])];
+                for (int t = 0; t < kTeamWidth && t < nr; ++t) v += mr.rs[o + t] * aq[spx(mr, mr.ri[o + t])];
             }
             al[q] = v;
         }
@@ -606,16 +611,16 @@ __device__ __forceinline__ void w_jact(const DeviceMech& mr, WaveAdjLDS<N>& L, d
         };
 #pragma unroll
         for (int q = 0; q < kTeamWidth; ++q)
-            if (q < nl) in.w -= mr.ls[o + q] * L.dNb[mr.li[o + q]];
+            if (q < nl) in.w -= mr.ls[o + q] * L.dNb[spx(mr, mr.li[o + q])];
 #pragma unroll
         for (int q = 0; q < kTeamWidth; ++q)
-            if (q < nr) in.w += mr.rs[o + q] * L.dNb[mr.ri[o + q]];
+            if (q < nr) in.w += mr.rs[o + q] * L.dNb[spx(mr, mr.ri[o + q])];
 #pragma unroll
         for (int j = 0; j < kTeamWidth; ++j)
-            if (j < nl) in.dCf[j] = u(mr.li[o + j]);
+            if (j < nl) in.dCf[j] = u(spx(mr, mr.li[o + j]));
 #pragma unroll
         for (int j = 0; j < kTeamWidth; ++j)
-            if (j < nr) in.dCr[j] = -u(mr.ri[o + j]);
+            if (j < nr) in.dCr[j] = -u(spx(mr, mr.ri[o + j]));
         in.dwdT = u(iT);
         if (mr.type[r] == GFC_RATE_THIRD_BODY) {
             double D = 0.0;
@@ -730,8 +735,9 @@ wave_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer,
     for (;;) {
         int c = 0;
         if (lane == 0) c = atomicAdd(queue, 1);
-        const int cell = __builtin_amdgcn_readfirstlane(__shfl(c, 0));
+        int cell = __builtin_amdgcn_readfirstlane(__shfl(c, 0));
         if (cell >= A.nCells) break;
+        cell = bound(cell, A.nCells, mr.bounds, kBoundCell);
         const double* y0c = rec_slot(R.y0c, cell, n + 1);
         const double p = y0c[(long)n * kRecStride];
         const double T0 = y0c[(long)nS * kRecStride];
@@ -745,14 +751,16 @@ wave_adjoint_reverse_kernel(SolveArgs A, T16Record R, double* seg, long segPer,
         if (T0 > A.Treact && A.deltaT > kSmall) {
             const double* yE = rec_slot(R.yEnd, cell, n);
             if (lane < nS && yE[lane * kRecStride] < 0.0) yb = 0.0;
-            const int nAcc = R.nAcc[cell];
+            const int nAcc = bound(R.nAcc[cell], R.maxSteps + 1, mr.bounds, kBoundStep);
             const double* ck = rec_slot(R.ck, cell, R.nck * n);
             const double* hs = rec_slot(R.hs, cell, R.maxSteps);
             double* sg = rec_slot(seg, cell, segPer);
             const int nseg = (nAcc + R.K - 1) / R.K;
             for (int g = nseg - 1; g >= 0; --g) {
                 const int s0 = g * R.K, s1 = min(s0 + R.K, nAcc);
-                double y = (lane < n) ? ck[((long)g * n + lane) * kRecStride] : 0.0;
+                double y = (lane < n) ? ck[((long)bound(g, R.nck, mr.bounds, kBoundSlot) * n + lane) *
+                                             kRecStride]
+                                        : 0.0;
                 if (lane < n) sg[lane * kRecStride] = y;
                 WStages<N> S;
                 for (int t = s0; t < s1 - 1; ++t) {

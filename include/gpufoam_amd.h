@@ -46,7 +46,8 @@ typedef enum gfc_status {
     GFC_ERR_CAPACITY = 9,        /* nCells > maxCells of the evaluator */
     GFC_ERR_HIP = 10,            /* HIP runtime error */
     GFC_ERR_NOT_BUILT = 11,      /* no kernel for this mechanism size */
-    GFC_ERR_LTS = 12             /* local time stepping is not supported */
+    GFC_ERR_LTS = 12,            /* local time stepping is not supported */
+    GFC_ERR_BOUNDS = 13          /* a bounds-check build caught an index out of range */
 } gfc_status;
 
 typedef enum gfc_rate_type {
@@ -151,6 +152,15 @@ typedef struct gfc_evaluator gfc_evaluator;
 
 /* Last error message of this thread ("" if none). */
 const char* gfc_last_error(void);
+
+/* 1 in the debug bounds-check build of the library (built with
+ * GFC_BOUNDS_CHECK: indices read from the tables, work-queue cells and the
+ * adjoint's recorded steps are checked before they address memory; a
+ * solve or adjoint call that catches one returns GFC_ERR_BOUNDS after
+ * waiting for its stream), 0 in the default build.  No counterpart in
+ * gpuFoam: a debugging aid for the batched kernels' shared-memory and
+ * work-stealing indexing. */
+int gfc_bounds_checked(void);
 
 /* ---- mechanism compiler (host) ----------------------------------------
  * Replaces makeGpuThermos (makeGpuThermo.H:16-66), makeGpuReactions and

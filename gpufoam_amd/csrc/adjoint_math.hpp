@@ -174,16 +174,17 @@ GFC_HD inline void mixture_reverse(const DeviceMech& m, double p, SV y, const Mi
 struct ArrBar {
     double T, lnA, beta, Ta;
 };
-GFC_HD __forceinline__ ArrBar arr_adjoint(const double* kk, double T, double lnT, double k,
+GFC_HD __forceinline__ ArrBar arr_adjoint(const double* kk, double iT, double lnT, double k,
                                           double kb, double kqb) {
+    // iT = 1/T: the adjoint's divisions by T are products with it
     const double be = kk[1], Ta = kk[2];
-    const double q = (be + Ta / T) / T;
-    const double dq = -(be + 2.0 * Ta / T) / (T * T);
+    const double q = (be + Ta * iT) * iT;
+    const double dq = -(be + 2.0 * Ta * iT) * (iT * iT);
     ArrBar a;
     a.T = kb * k * q + kqb * (k * q * q + k * dq);
     a.lnA = kb * k + kqb * k * q;
-    a.beta = kb * k * lnT + kqb * (k * lnT * q + k / T);
-    a.Ta = -kb * k / T + kqb * (k / (T * T) - k * q / T);
+    a.beta = kb * k * lnT + kqb * (k * lnT * q + k * iT);
+    a.Ta = -kb * k * iT + kqb * (k * (iT * iT) - k * q * iT);
     return a;
 }
 
@@ -229,10 +230,12 @@ GFC_HD inline KcVal kc_eval(const DeviceMech& m, int r, double T) {
     KcVal v;
     KcTerms t;
     reaction_Kc_dKc(m, r, T, v.Kc, v.dK, &t);
-    const double HT = t.S + t.Gstd / T; // H/T
-    v.dlnKc = (t.arg < 600 ? HT * t.Y / t.RRT : 0.0) - (t.nmSmall ? 0.0 : t.nm / T);
-    v.ddK = (t.Y / t.RRT) * (poly_Cp(t.a, T) / T - 2.0 * HT / T) +
-            (t.nmSmall ? 0.0 : t.nm / (T * T));
+    const double iT = 1.0 / T;
+    const double HT = t.S + t.Gstd * iT; // H/T
+    const double YR = t.Y / t.RRT;
+    v.dlnKc = (t.arg < 600 ? HT * YR : 0.0) - (t.nmSmall ? 0.0 : t.nm * iT);
+    v.ddK = YR * (poly_Cp(t.a, T) * iT - 2.0 * HT * iT) +
+            (t.nmSmall ? 0.0 : t.nm * (iT * iT));
     return v;
 }
 
@@ -344,7 +347,8 @@ GFC_HD __forceinline__ void reaction_reverse_w(const DeviceMech& m, int r, doubl
     if (!MSET && t != GFC_RATE_ARRHENIUS)
         for (int i = 0; i < m.nS; ++i) M += eff[i] * c[i];
     const double kA = arrhenius(kk, Te);
-    const double qA = (kk[1] + kk[2] / Te) / Te;
+    const double iTe = 1.0 / Te;
+    const double qA = (kk[1] + kk[2] * iTe) * iTe;
     double kf, dkfdT, kI = 0.0, qI = 0.0, temp = 0.0, g = 0.0, Pr = 0.0;
     TroeVal F{1.0, 0.0, 0.0};
     const double* ki = m.kInf + 3 * r;
@@ -356,7 +360,7 @@ GFC_HD __forceinline__ void reaction_reverse_w(const DeviceMech& m, int r, doubl
         dkfdT = M * (kA * qA);
     } else {
         kI = arrhenius(ki, Te);
-        qI = (ki[1] + ki[2] / Te) / Te;
+        qI = (ki[1] + ki[2] * iTe) * iTe;
         Pr = kA / kI * M;
         if (t == GFC_RATE_TROE) F = troe_eval(m.troe + 4 * r, Te, Pr);
         g = Pr / (1.0 + Pr);
@@ -375,8 +379,9 @@ GFC_HD __forceinline__ void reaction_reverse_w(const DeviceMech& m, int r, doubl
         K = kc_eval(m, r, Te);
     }
     const double Cf = L.C;
-    const double kr = rev ? kf / K.Kc : 0.0;
-    const double dkrdT = rev ? dkfdT / K.Kc - kr * K.dK : 0.0;
+    const double iKc = 1.0 / K.Kc;
+    const double kr = rev ? kf * iKc : 0.0;
+    const double dkrdT = rev ? dkfdT * iKc - kr * K.dK : 0.0;
 
     double Cfb = in.w * kf, Crb = -in.w * kr, kfb = in.w * Cf, krb = -in.w * Cr;
     double dkfdTb = 0.0, dkrdTb = 0.0, Kcb = 0.0, kAb = 0.0;
@@ -402,19 +407,19 @@ GFC_HD __forceinline__ void reaction_reverse_w(const DeviceMech& m, int r, doubl
                 }
         if (t == GFC_RATE_THIRD_BODY) {
             Cfb += in.D * kA;
-            Crb += -in.D * kA / K.Kc;
-            Kcb += in.D * kA * Cr / (K.Kc * K.Kc);
-            kAb += in.D * (Cf - Cr / K.Kc);
+            Crb += -in.D * kA * iKc;
+            Kcb += in.D * kA * Cr * (iKc * iKc);
+            kAb += in.D * (Cf - Cr * iKc);
         }
     }
     double TbE = 0.0;
     if (rev) {
-        dkfdTb += dkrdTb / K.Kc;
-        Kcb += -dkrdTb * dkfdT / (K.Kc * K.Kc);
+        dkfdTb += dkrdTb * iKc;
+        Kcb += -dkrdTb * dkfdT * (iKc * iKc);
         krb += -dkrdTb * K.dK;
         const double dKb = -dkrdTb * kr;
-        kfb += krb / K.Kc;
-        Kcb += -krb * kr / K.Kc;
+        kfb += krb * iKc;
+        Kcb += -krb * kr * iKc;
         TbE += Kcb * K.Kc * K.dlnKc + dKb * K.ddK;
     }
     L.reverse(c, Cfb, dCLb, cbar);
@@ -422,10 +427,10 @@ GFC_HD __forceinline__ void reaction_reverse_w(const DeviceMech& m, int r, doubl
     double Mb = 0.0;
     ArrBar a0{0.0, 0.0, 0.0, 0.0}, a1{0.0, 0.0, 0.0, 0.0};
     if (t == GFC_RATE_ARRHENIUS) {
-        a0 = arr_adjoint(kk, Te, lnT, kA, kfb, dkfdTb);
+        a0 = arr_adjoint(kk, iTe, lnT, kA, kfb, dkfdTb);
     } else if (t == GFC_RATE_THIRD_BODY) {
         Mb = kfb * kA + dkfdTb * (kA * qA);
-        a0 = arr_adjoint(kk, Te, lnT, kA, kfb * M + kAb, dkfdTb * M);
+        a0 = arr_adjoint(kk, iTe, lnT, kA, kfb * M + kAb, dkfdTb * M);
     } else {
         double kIb = kfb * temp;
         const double tempb = kfb * kI + dkfdTb * (kI * qI);
@@ -433,11 +438,12 @@ GFC_HD __forceinline__ void reaction_reverse_w(const DeviceMech& m, int r, doubl
         const double gp = 1.0 / ((1.0 + Pr) * (1.0 + Pr));
         const double Prb = tempb * (gp * F.F + g * F.dPr);
         TbE += tempb * g * F.dT;
-        const double k0b = Prb * M / kI;
-        Mb = Prb * kA / kI;
-        kIb += -Prb * Pr / kI;
-        a0 = arr_adjoint(kk, Te, lnT, kA, k0b, 0.0);
-        a1 = arr_adjoint(ki, Te, lnT, kI, kIb, kqIb);
+        const double ikI = 1.0 / kI;
+        const double k0b = Prb * M * ikI;
+        Mb = Prb * kA * ikI;
+        kIb += -Prb * Pr * ikI;
+        a0 = arr_adjoint(kk, iTe, lnT, kA, k0b, 0.0);
+        a1 = arr_adjoint(ki, iTe, lnT, kI, kIb, kqIb);
     }
     TbE += a0.T;
     TbE += a1.T;

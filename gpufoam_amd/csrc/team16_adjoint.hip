@@ -197,22 +197,38 @@ template <int N>
 __device__ __forceinline__ double t16_solveT(const T16AdjLDS<N>& L, int n, int ps, double r,
                                              int lane, int mem) {
     constexpr int P = T16AdjLDS<N>::P;
-    // U^T w = r
+    // U^T w = r; L.f.gH holds the pivots' reciprocals (t16_pivot_reciprocals)
     double w = 0.0;
-    for (int j = 0; j < n; ++j) {
-        const double wj = team_bcast(r, lane, j) / L.LU()[j * P + j];
-        if (mem == j) w = wj;
-        if (mem > j && mem < n) r -= L.LU()[j * P + mem] * wj;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        if (j < n) {
+            const double wj = team_bcast_k(r, j) * L.f.gH[j];
+            if (mem == j) w = wj;
+            if (mem > j && mem < n) r -= L.LU()[j * P + mem] * wj;
+        }
     }
     // L^T v = w (unit diagonal)
     double v = 0.0;
-    for (int k = n - 1; k >= 0; --k) {
-        const double vk = team_bcast(w, lane, k);
-        if (mem == k) v = vk;
-        if (mem < k) w -= L.LU()[k * P + mem] * vk;
+#pragma unroll
+    for (int k = N - 1; k >= 0; --k) {
+        if (k < n) {
+            const double vk = team_bcast_k(w, k);
+            if (mem == k) v = vk;
+            if (mem < k) w -= L.LU()[k * P + mem] * vk;
+        }
     }
     // x = P^T v: row i sits at position ps_i
     return team_bcast(v, lane, ps);
+}
+
+// 1 / U[j][j] of the step's factors into L.f.gH (the specie Gibbs array of
+// the forward evaluations, unused until the next step's recompute).
+template <int N>
+__device__ __forceinline__ void t16_pivot_reciprocals(T16AdjLDS<N>& L, int n, int mem) {
+    static_assert(N <= kTW, "a pivot per member");
+    constexpr int P = T16AdjLDS<N>::P;
+    if (mem < n) L.f.gH[mem] = 1.0 / L.LU()[mem * P + mem];
+    wave_sync();
 }
 
 // Jb(i, j) += r_i k_j
@@ -518,6 +534,7 @@ __device__ __forceinline__ double t16_step_reverse(const SolveArgs& A, const Dev
     const bool inRange = T0 >= m.Tlow && T0 <= m.Thigh;
     T16Stages S;
     (void)t16_step<N, true>(A, L, p, y0, h, lane, mem, S);
+    t16_pivot_reciprocals<N>(L, n, mem);
     double kb1 = t.b1 * yb, kb2 = t.b2 * yb, kb3 = t.b3 * yb, kb4 = t.b4 * yb;
     double Fb = 0.0, fb2 = 0.0, fb3 = 0.0;
     for (int e = mem; e < N * T16AdjLDS<N>::P; e += kTW) L.Jb()[e] = 0.0;

@@ -62,6 +62,47 @@ __device__ __forceinline__ int team_bcast_i(int v, int lane, int src) {
     return __shfl(v, (lane & ~15) | src);
 }
 
+// Member S's value to its whole team by DPP row_newbcast (a VALU move, no
+// LDS round trip).  Every lane of the team must be active.
+template <int S>
+__device__ __forceinline__ int team_bcast_ci(int v) {
+    static_assert(S >= 0 && S < 16, "a member of the team");
+    return __builtin_amdgcn_update_dpp(0, v, 0x150 + S, 0xF, 0xF, false);
+}
+template <int S>
+__device__ __forceinline__ double team_bcast_c(double v) {
+    const int lo = team_bcast_ci<S>(__double2loint(v));
+    const int hi = team_bcast_ci<S>(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+// The same for a member index known at compile time once the caller's loop
+// is unrolled (the switch folds to one case).
+__device__ __forceinline__ int team_bcast_ki(int v, int j) {
+    switch (j) {
+    case 0: return team_bcast_ci<0>(v);
+    case 1: return team_bcast_ci<1>(v);
+    case 2: return team_bcast_ci<2>(v);
+    case 3: return team_bcast_ci<3>(v);
+    case 4: return team_bcast_ci<4>(v);
+    case 5: return team_bcast_ci<5>(v);
+    case 6: return team_bcast_ci<6>(v);
+    case 7: return team_bcast_ci<7>(v);
+    case 8: return team_bcast_ci<8>(v);
+    case 9: return team_bcast_ci<9>(v);
+    case 10: return team_bcast_ci<10>(v);
+    case 11: return team_bcast_ci<11>(v);
+    case 12: return team_bcast_ci<12>(v);
+    case 13: return team_bcast_ci<13>(v);
+    case 14: return team_bcast_ci<14>(v);
+    default: return team_bcast_ci<15>(v);
+    }
+}
+__device__ __forceinline__ double team_bcast_k(double v, int j) {
+    const int lo = team_bcast_ki(__double2loint(v), j);
+    const int hi = team_bcast_ki(__double2hiint(v), j);
+    return __hiloint2double(hi, lo);
+}
+
 // Sum over the team in a fixed tree (row_shr 1, 2, 4, 8); every member gets
 // the value of the row's last lane.
 __device__ __forceinline__ double team_sum(double v, int lane) {
@@ -69,7 +110,7 @@ __device__ __forceinline__ double team_sum(double v, int lane) {
     v += dpp<0x112>(0.0, v);
     v += dpp<0x114>(0.0, v);
     v += dpp<0x118>(0.0, v);
-    return team_bcast(v, lane, 15);
+    return team_bcast_c<15>(v);
 }
 
 __device__ __forceinline__ double team_max(double v, int lane) {
@@ -78,7 +119,7 @@ __device__ __forceinline__ double team_max(double v, int lane) {
     v = fmax(v, dpp<0x112>(lo, v));
     v = fmax(v, dpp<0x114>(lo, v));
     v = fmax(v, dpp<0x118>(lo, v));
-    return team_bcast(v, lane, 15);
+    return team_bcast_c<15>(v);
 }
 
 __device__ __forceinline__ int team_max_i(int v, int lane) {
@@ -87,7 +128,7 @@ __device__ __forceinline__ int team_max_i(int v, int lane) {
     v = max(v, dppi<0x112>(lo, v));
     v = max(v, dppi<0x114>(lo, v));
     v = max(v, dppi<0x118>(lo, v));
-    return team_bcast_i(v, lane, 15);
+    return team_bcast_ci<15>(v);
 }
 
 template <int N>
@@ -413,14 +454,14 @@ __device__ __forceinline__ double t16_solve(const double (&row)[N], int lane, in
                                             double rd, int plv, double b) {
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        const int pv = team_bcast_i(plv, lane, j);
+        const int pv = team_bcast_ki(plv, j);
         const double yj = team_bcast(b, lane, pv);
         if (ps > j) b = fma(-row[j], yj, b);
     }
     double x = 0.0;
 #pragma unroll
     for (int j = N - 1; j >= 0; --j) {
-        const int pv = team_bcast_i(plv, lane, j);
+        const int pv = team_bcast_ki(plv, j);
         const double xj = team_bcast(b * rd, lane, pv);
         if (ps < j) b = fma(-row[j], xj, b);
         if (mem == j) x = xj;

@@ -53,7 +53,8 @@ struct T16AdjLDS {
     // the factors and Jbar only after it, until the step's reverse ends.
     static constexpr bool kShare = 2 * N * P <= kSets16 * kTW;
     Team16LDS<N> f;     // the solve's arrays; f.J holds the step's Jacobian
-    double raw[N * P];  // species rows before the transform
+    double raw[2 * kTW]; // per species row before the transform: sum_j
+                         // (dN_i/dc_j) c_j, then dN_i/dT
     double own[kShare ? 1 : 2 * N * P];
     double cb[kTW];     // concentration adjoints
     double dNb[kTW];    // adjoints of dN_i/dt
@@ -418,11 +419,16 @@ __device__ __forceinline__ void t16_jact(const DeviceMech& mr, T16AdjLDS<N>& L, 
     // row transforms: vb_j (species columns) and the temperature column's
     // concentration adjoints.
     if (sp) {
+        // sum_i g_i (W_i/rho) t_im with t the rows before the transform:
+        // the transform made J_im = (W_i/rho) t_im (rho/W_m) + rho v_m F_i
+        double gJ = 0.0, gF = 0.0;
         for (int i = 0; i < nS; ++i) {
             vb += L.Jb()[i * P + mem] * rho * L.v[i];
             const double g = L.Jb()[i * P + iT];
-            cbT += -g * (mr.W[i] / rho) * L.raw[i * P + mem] * av;
+            gJ += g * L.f.J[i * P + mem];
+            gF += g * L.v[i];
         }
+        cbT = -(gJ - (rho * x.vi) * gF) / (rho / mr.W[mem]) * av;
     }
     wave_sync();
     double rhoPart = 0.0, avPart = 0.0;
@@ -437,9 +443,8 @@ __device__ __forceinline__ void t16_jact(const DeviceMech& mr, T16AdjLDS<N>& L, 
         }
         // temperature column
         const double g = row[iT];
-        double sumc = 0.0;
-        for (int j = 0; j < nS; ++j) sumc += L.raw[mem * P + j] * L.f.c[j];
-        const double X = L.raw[mem * P + iT] - sumc * av;
+        const double sumc = L.raw[mem];
+        const double X = L.raw[kTW + mem] - sumc * av;
         row[iT] = g * WiR;
         for (int j = 0; j < nS; ++j) row[j] += -g * WiR * L.f.c[j] * av;
         avPart = g * (F - WiR * sumc);

@@ -215,8 +215,10 @@ __device__ __forceinline__ double t16_derivatives(const DeviceMech& m, double p,
 // leaves J in L.J and returns this member's component of the derivative
 // computed alongside (dfdx).  The mechanism's efficiency sets (at most 16)
 // are added row-parallel in the transform, as in the 64-lane kernel.  With
-// RAW, the species rows before the transform (dN_i/dc_j with the dense
-// third-body part, and dN_i/dT) are also stored to raw[i*P + j].
+// RAW, two quantities of each species row before the transform (dN_i/dc_j
+// with the dense third-body part, and dN_i/dT) are also stored:
+// raw[i] = sum_j (dN_i/dc_j) c_j, summed in column order, and
+// raw[16 + i] = dN_i/dT.
 template <int N, bool RAW = false>
 __device__ __forceinline__ double t16_jacobian(const DeviceMech& m, double p, double T,
                                                double yl, Team16LDS<N>& L, int lane, int mem,
@@ -361,7 +363,8 @@ __device__ __forceinline__ double t16_jacobian(const DeviceMech& m, double p, do
         double* row = &L.J[mem * P];
         const double WiByrhoM = m.W[mem] / rhoM;
         double ddNdT = row[nS];
-        if constexpr (RAW) raw[mem * P + nS] = ddNdT;
+        if constexpr (RAW) raw[kTW + mem] = ddNdT;
+        double sc = 0.0;
         double w[kSetsT];
 #pragma unroll
         for (int q = 0; q < kSetsT; ++q) w[q] = L.Wd[q * kTW + mem];
@@ -372,11 +375,12 @@ __device__ __forceinline__ double t16_jacobian(const DeviceMech& m, double p, do
 #pragma unroll
             for (int q = 0; q < kSetsT; ++q) acc += w[q] * e[q];
             const double tj = row[j] + acc;
-            if constexpr (RAW) raw[mem * P + j] = tj;
+            if constexpr (RAW) sc += tj * L.c[j];
             ddNdT -= tj * L.c[j] * alphavM;
             row[j] = WiByrhoM * tj * L.colA[j] + L.colB[j] * d;
         }
         row[nS] = WiByrhoM * ddNdT + alphavM * d;
+        if constexpr (RAW) raw[mem] = sc;
     }
     wave_sync();
     // ddT_dtdY (:108-126): member i sums column i

@@ -33,9 +33,12 @@ DEV = "cuda"
 
 
 def _case(config, ncells, maxSteps=None):
-    if config == "syn60s":
-        t, r = syn_texts(seed=13, nS=60, nR=45)
-        m = Mechanism(t, r, "syn60s")
+    if config in ("syn60s", "syn60m"):
+        # syn60m: 90 reactions with 27 distinct third-body efficiency sets,
+        # more than the 16 whose products with the Jb factors the reverse
+        # precomputes (the per-specie path of D is taken)
+        t, r = syn_texts(seed=13, nS=60, nR=45 if config == "syn60s" else 90)
+        m = Mechanism(t, r, config)
         Y0, dt0 = syn_states(m, ncells, seed=14)
         # every specie present (the generator leaves most at zero, where
         # max(Y, 0) makes the map non-smooth), renormalised
@@ -75,7 +78,8 @@ def _backward(ev, m, W):
     return Y0bar.cpu().numpy(), pbar.cpu().numpy()
 
 
-@pytest.mark.parametrize("config,ncells", [("gri_production", 256), ("syn60s", 32)])
+@pytest.mark.parametrize("config,ncells",
+                         [("gri_production", 256), ("syn60s", 32), ("syn60m", 32)])
 def test_recording_solve_is_the_wavefront_solve(config, ncells):
     m, Y0, dt0, c, ode = _case(config, ncells)
     print(f"case {config} n={ncells} record vs batched, wavefront per cell", flush=True)
@@ -89,7 +93,8 @@ def test_recording_solve_is_the_wavefront_solve(config, ncells):
     assert np.array_equal(st.cpu().numpy(), stb)
 
 
-@pytest.mark.parametrize("config,ncells", [("gri_production", 64), ("syn60s", 32)])
+@pytest.mark.parametrize("config,ncells",
+                         [("gri_production", 64), ("syn60s", 32), ("syn60m", 32)])
 def test_wave_gradients_equal_lane_gradients(config, ncells):
     m, Y0, dt0, c, ode = _case(config, ncells)
     n = m.n_species + 1

@@ -157,6 +157,7 @@ __device__ __forceinline__ double t16_step(const SolveArgs& A, T16AdjLDS<N>& L, 
                 if (k < n) L.LU()[ps * P + k] = row[k];
         }
     }
+    t16_pivot_order<N>(row, rd, lane, plv);
     double k1 = 0, k2 = 0, k3 = 0, k4 = 0, dydx = 0, f2 = 0, f3 = 0;
 #pragma unroll 1
     for (int st = 0; st < tb.stages; ++st) {
@@ -171,7 +172,7 @@ __device__ __forceinline__ double t16_step(const SolveArgs& A, T16AdjLDS<N>& L, 
         else if (st == 1) rhs = dydx + dx * tb.d2 * dfdx + tb.c21 * k1 / dx;
         else if (st == 2) rhs = dydx + dx * tb.d3 * dfdx + (tb.c31 * k1 + tb.c32 * k2) / dx;
         else rhs = dydx + dx * tb.d4 * dfdx + (tb.c41 * k1 + tb.c42 * k2 + tb.c43 * k3) / dx;
-        const double kx = t16_solve<N>(row, lane, mem, ps, rd, plv, rhs);
+        const double kx = t16_solve<N>(row, lane, mem, rd, plv, rhs);
         if (st == 0) k1 = kx;
         else if (st == 1) k2 = kx;
         else if (st == 2) k3 = kx;

@@ -453,21 +453,33 @@ __device__ __forceinline__ void t16_lu(double (&row)[N], int n, int lane, int me
     }
 }
 
+// Factors into pivot order: member m takes the row (and the reciprocal
+// pivot) at pivot position m from the member holding it (plv), so that the
+// solves broadcast from fixed members (DPP) instead of data-dependent ones.
 template <int N>
-__device__ __forceinline__ double t16_solve(const double (&row)[N], int lane, int mem, int ps,
-                                            double rd, int plv, double b) {
+__device__ __forceinline__ void t16_pivot_order(double (&row)[N], double& rd, int lane, int plv) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) row[k] = team_bcast(row[k], lane, plv);
+    rd = team_bcast(rd, lane, plv);
+}
+
+// Solves (LU) x = b with the factors in pivot order (t16_pivot_order); b
+// and x species-parallel.  The same operations as the solve over the rows
+// in place, member m now standing for the row at pivot position m.
+template <int N>
+__device__ __forceinline__ double t16_solve(const double (&row)[N], int lane, int mem, double rd,
+                                            int plv, double b) {
+    double bp = team_bcast(b, lane, plv); // the right-hand side of position mem
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        const int pv = team_bcast_ki(plv, j);
-        const double yj = team_bcast(b, lane, pv);
-        if (ps > j) b = fma(-row[j], yj, b);
+        const double yj = team_bcast_k(bp, j);
+        if (mem > j) bp = fma(-row[j], yj, bp);
     }
     double x = 0.0;
 #pragma unroll
     for (int j = N - 1; j >= 0; --j) {
-        const int pv = team_bcast_ki(plv, j);
-        const double xj = team_bcast(b * rd, lane, pv);
-        if (ps < j) b = fma(-row[j], xj, b);
+        const double xj = team_bcast_k(bp * rd, j);
+        if (mem < j) bp = fma(-row[j], xj, bp);
         if (mem == j) x = xj;
     }
     return x;

@@ -108,6 +108,7 @@ __device__ __forceinline__ void t16_run(const SolveArgs& A, int* queue, const T1
                         int ps, plv;
                         double rd;
                         t16_lu<N>(row, n, lane, mem, ps, rd, plv);
+                        t16_pivot_order<N>(row, rd, lane, plv);
                         // Rosenbrock stages (gpuRosenbrock{12,23,34}.H::solve)
                         double k1 = 0, k2 = 0, k3 = 0, k4 = 0, dydx = 0;
 #pragma unroll 1
@@ -123,7 +124,7 @@ __device__ __forceinline__ void t16_run(const SolveArgs& A, int* queue, const T1
                             else if (st == 1) rhs = dydx + dx * tb.d2 * dfdx + tb.c21 * k1 / dx;
                             else if (st == 2) rhs = dydx + dx * tb.d3 * dfdx + (tb.c31 * k1 + tb.c32 * k2) / dx;
                             else rhs = dydx + dx * tb.d4 * dfdx + (tb.c41 * k1 + tb.c42 * k2 + tb.c43 * k3) / dx;
-                            const double kx = t16_solve<N>(row, lane, mem, ps, rd, plv, rhs);
+                            const double kx = t16_solve<N>(row, lane, mem, rd, plv, rhs);
                             if (st == 0) k1 = kx;
                             else if (st == 1) k2 = kx;
                             else if (st == 2) k3 = kx;
@@ -256,8 +257,9 @@ __global__ void __launch_bounds__(64) team16_eval_kernel(EvalArgs E) {
         int ps, plv;
         double rd;
         t16_lu<N>(row, n, lane, mem, ps, rd, plv);
+        t16_pivot_order<N>(row, rd, lane, plv);
         const double bl = (mem < n) ? E.b[mem * ld + cell] : 0.0;
-        const double xl = t16_solve<N>(row, lane, mem, ps, rd, plv, bl);
+        const double xl = t16_solve<N>(row, lane, mem, rd, plv, bl);
         if (mem < n) E.x[mem * ld + cell] = xl;
         return;
     }

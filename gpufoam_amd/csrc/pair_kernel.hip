@@ -219,11 +219,10 @@ __device__ __forceinline__ double pair_jacobian(const DeviceMech& m, double p, d
             const double Cf = side_powers(x.li, x.lei, x.le, x.nl, L.c, pwl);
             dwdT = Cf * ddT;
             w_ = Cf * kf;
-            double Cr = 0.0, Kc = 1.0;
+            double Cr = 0.0, lnKc = 0.0;
             if (!x.irr) {
                 double dKc;
-                const double lnKc = ln_kc(x, f, L, &dKc);
-                Kc = exp(lnKc);
+                lnKc = ln_kc(x, f, L, &dKc);
                 const double rKc = exp(-lnKc);
                 kr = kf * rKc;
                 Cr = side_powers(x.ri, x.rei, x.re, x.nr, L.c, pwr);
@@ -231,7 +230,10 @@ __device__ __forceinline__ double pair_jacobian(const DeviceMech& m, double p, d
                 dwdT -= Cr * dkrdT;
                 w_ -= Cr * kr;
             }
-            const double temp = (Cf - Cr / Kc);
+            // Cf - Cr/Kc: used by the third-body and fall-off rates only, so the
+            // Arrhenius passes skip its exponential
+            double temp = Cf;
+            if (x.type != GFC_RATE_ARRHENIUS) temp = Cf - Cr / exp(lnKc);
             g = (x.type == GFC_RATE_THIRD_BODY) ? kArr * temp : 0.0;
             if (x.type > GFC_RATE_THIRD_BODY && !isfinite(temp)) poisoned = true;
             // dN/dt (the reference's dfdx): per-wavefront sums

@@ -268,11 +268,10 @@ __device__ __forceinline__ double t16_jacobian(const DeviceMech& m, double p, do
         const double Cf = side_powers(x.li, x.lei, x.le, x.nl, L.c, pwl);
         double dwdT = Cf * ddT;
         double w = Cf * kf;
-        double Cr = 0.0, Kc = 1.0, kr = 0.0;
+        double Cr = 0.0, lnKc = 0.0, kr = 0.0;
         if (!x.irr) {
             double dKc;
-            const double lnKc = ln_kc(x, f, L, &dKc);
-            Kc = exp(lnKc);
+            lnKc = ln_kc(x, f, L, &dKc);
             const double rKc = exp(-lnKc);
             kr = kf * rKc;
             Cr = side_powers(x.ri, x.rei, x.re, x.nr, L.c, pwr);
@@ -280,7 +279,10 @@ __device__ __forceinline__ double t16_jacobian(const DeviceMech& m, double p, do
             dwdT -= Cr * dkrdT;
             w -= Cr * kr;
         }
-        const double temp = (Cf - Cr / Kc);
+        // Cf - Cr/Kc: used by the third-body and fall-off rates only, so the
+        // Arrhenius passes skip its exponential
+        double temp = Cf;
+        if (x.type != GFC_RATE_ARRHENIUS) temp = Cf - Cr / exp(lnKc);
         if (x.type == GFC_RATE_THIRD_BODY) {
             const double g = kArr * temp;
             double* Wd = &L.Wd[x.set * kTW];

@@ -468,14 +468,13 @@ int gfc_adjoint_forward(gfc_evaluator* ev, double deltaT, int nCells, int ldY,
     if (st->waveN) {
         SolveArgs At = A;
         At.m = ev->dmTeam;
-        At.qChunk = 1;  // one cell per work-queue ticket
-        At.qBulk = 0;
         const t16::T16Record R = record_of(st, layout(ev));
         auto k = st->waveN == 32 ? team_record_kernel<32>
                : st->waveN == 54 ? team_record_kernel<54> : team_record_kernel<64>;
+        const int grid = resident_grid(st, ev->device, k, nCells, 1);
+        queue_chunks_for(ev, At, grid); // the batched solve's tickets
         hipLaunchKernelGGL(adj_reset_queue_kernel, dim3(1), dim3(1), 0, s, st->queue);
-        hipLaunchKernelGGL(k, dim3(resident_grid(st, ev->device, k, nCells, 1)), dim3(64), 0,
-                           s, At, st->queue, R);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, At, st->queue, R);
     } else if (st->teamN) {
         SolveArgs At = A;
         At.m = ev->dmTeam;

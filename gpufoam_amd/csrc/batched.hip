@@ -143,6 +143,10 @@ bool hipFail(hipError_t e, const char* what) {
 
 } // namespace
 
+void queue_chunks_for(const gfc_evaluator* ev, SolveArgs& A, int waves) {
+    set_queue_chunks(ev, A, waves);
+}
+
 int batched_init(gfc_evaluator* ev) {
     auto* b = new BatchedState();
     ev->batched = b;

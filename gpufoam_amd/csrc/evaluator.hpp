@@ -75,4 +75,7 @@ bool team16_selected(const gfc_evaluator* ev, int nCells, int& N);
 // evaluator's mechanism (0: none; nSpecie+1 <= 64, at most 3 species per
 // reaction side and 128 efficiency sets, gpuFoam's rate types only).
 int wave_kernel_rows(const gfc_evaluator* ev);
+// The work-queue chunking of the persistent wavefront kernel for a grid of
+// `waves` wavefronts (16-cell tickets, the last cells one by one).
+void queue_chunks_for(const gfc_evaluator* ev, SolveArgs& A, int waves);
 } // namespace gfc

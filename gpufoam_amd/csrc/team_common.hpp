@@ -118,12 +118,20 @@ __device__ __forceinline__ Rxn load_rxn(const DeviceMech& m, int pass, int lane)
     x.beta = D[rdBeta * kLanes];
     x.Ta = D[rdTa * kLanes];
     x.sgnA = D[rdSgnA * kLanes];
-    x.lnAi = D[rdLnAi * kLanes];
-    x.betai = D[rdBetai * kLanes];
-    x.Tai = D[rdTai * kLanes];
-    x.sgnAi = D[rdSgnAi * kLanes];
+    // the fall-off fields only for the fall-off rates (a pass of Arrhenius
+    // or third-body reactions does not load them)
+    x.lnAi = x.betai = x.Tai = 0.0;
+    x.sgnAi = 1.0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) x.troe[q] = D[(rdAlpha + q) * kLanes];
+    for (int q = 0; q < 4; ++q) x.troe[q] = 0.0;
+    if (x.type > GFC_RATE_THIRD_BODY) {
+        x.lnAi = D[rdLnAi * kLanes];
+        x.betai = D[rdBetai * kLanes];
+        x.Tai = D[rdTai * kLanes];
+        x.sgnAi = D[rdSgnAi * kLanes];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x.troe[q] = D[(rdAlpha + q) * kLanes];
+    }
     x.dn = D[rdDn * kLanes];
     x.Tcommon = D[rdTcommon * kLanes];
     if (kBoundsChecked) {
